@@ -1,0 +1,728 @@
+// C ABI of the gfx950 PBF backend (declared in include/fluid_hip.h).
+//
+// Replaces the reference's CUDA entry points (cuda/include/fluid/cuda.h:7-9,
+// cuda/src/cuda_stub.cu:736-1099) with a device-resident context: state is
+// uploaded once, stepped in place for any number of substeps and downloaded
+// on request, instead of the reference's allocate/copy-in/copy-out per step.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fluid_hip.h"
+#include "pbf_ctx.h"
+
+using fluidhip::Ctx;
+
+struct fluid_hip_ctx {
+  Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(FLUID_HIP_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                    \
+  do {                                         \
+    hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// poly6 on the host, as core/src/core.cpp:35-46 evaluates it.
+float host_poly6(float r2, float h) {
+  const float h2 = h * h;
+  if (r2 > h2) return 0.0f;
+  const float diff = h2 - r2;
+  const float diff3 = diff * diff * diff;
+  const float h4 = h2 * h2;
+  const float h9 = h4 * h4 * h;
+  const float coeff = 315.0f / (64.0f * kPi * h9);
+  return coeff * diff3;
+}
+
+bool finite_all(const fluid_hip_params& p) {
+  const float v[] = {p.dt, p.density, p.particle_mass, p.h, p.epsilon, p.scorr_k,
+                     p.scorr_dq_coeff, p.visc_c, p.plane_restitution, p.plane_friction,
+                     p.vort_epsilon, p.vort_norm_eps, p.gravity[0], p.gravity[1], p.gravity[2]};
+  for (float x : v)
+    if (!std::isfinite(x)) return false;
+  for (int i = 0; i < p.n_planes && i < FLUID_HIP_MAX_PLANES; ++i)
+    for (int k = 0; k < 4; ++k)
+      if (!std::isfinite(p.planes[i][k])) return false;
+  return true;
+}
+
+// Fields every scene of a batch must share.
+bool same_shared(const fluid_hip_params& a, const fluid_hip_params& b) {
+  return a.dt == b.dt && a.density == b.density && a.particle_mass == b.particle_mass &&
+         a.h == b.h && a.epsilon == b.epsilon && a.solver_iterations == b.solver_iterations &&
+         a.use_uniform_grid == b.use_uniform_grid && a.enable_scorr == b.enable_scorr &&
+         a.enable_xsph == b.enable_xsph && a.enable_vorticity == b.enable_vorticity &&
+         a.scorr_k == b.scorr_k && a.scorr_n == b.scorr_n &&
+         a.scorr_dq_coeff == b.scorr_dq_coeff && a.visc_c == b.visc_c &&
+         a.vort_epsilon == b.vort_epsilon && a.vort_norm_eps == b.vort_norm_eps;
+}
+
+int validate(int n_scenes, const fluid_hip_params* params) {
+  if (!params) return fail(FLUID_HIP_EINVAL, "params is null");
+  for (int s = 0; s < n_scenes; ++s) {
+    const fluid_hip_params& p = params[s];
+    if (!finite_all(p)) return fail(FLUID_HIP_EINVAL, "non-finite parameter");
+    if (p.use_uniform_grid == 0)
+      return fail(FLUID_HIP_EUNSUPPORTED,
+                  "use_uniform_grid=0 (the O(N^2) neighbor path) is CPU-only");
+    if (!(p.h > 0.0f))
+      return fail(FLUID_HIP_EUNSUPPORTED, "h <= 0 selects the CPU-only O(N^2) path");
+    if (!(p.dt > 0.0f)) return fail(FLUID_HIP_EINVAL, "dt must be > 0");
+    if (!(p.density > 0.0f)) return fail(FLUID_HIP_EINVAL, "density must be > 0");
+    if (p.solver_iterations < 0) return fail(FLUID_HIP_EINVAL, "solver_iterations < 0");
+    if (p.n_planes < 0 || p.n_planes > FLUID_HIP_MAX_PLANES)
+      return fail(FLUID_HIP_EINVAL, "n_planes out of range");
+    if (s > 0 && !same_shared(params[0], p))
+      return fail(FLUID_HIP_EINVAL,
+                  "scenes of a batch may differ only in gravity, planes, restitution "
+                  "and friction");
+  }
+  return FLUID_HIP_OK;
+}
+
+void compute_consts(Ctx& c) {
+  // The loop invariants of core/src/core.cpp:138-148 and :270-274, in the
+  // same float expressions.
+  const fluid_hip_params& p = c.params[0];
+  fluidhip::StepConsts k{};
+  k.n = c.n;
+  k.n_scenes = c.n_scenes;
+  k.dt = p.dt;
+  const float h = p.h;
+  k.h = h;
+  k.inv_h = 1.0f / h;
+  const float h2 = h * h;
+  k.h2 = h2;
+  const float min_r = 0.01f * h;
+  k.min_r2 = min_r * min_r;
+  const float h4 = h2 * h2;
+  const float h9 = h4 * h4 * h;
+  k.w_coeff = 315.0f / (64.0f * kPi * h9);
+  k.w0 = host_poly6(0.0f, h);
+  const float h6 = h2 * h2 * h2;
+  k.s_coeff = -45.0f / (kPi * h6);
+  k.inv_density = 1.0f / p.density;
+  k.mass = p.particle_mass;
+  k.grad_scale = p.particle_mass * k.inv_density;
+  k.eps = p.epsilon;
+  const bool scorr_enabled = p.enable_scorr && p.scorr_k != 0.0f;
+  const float dq_coeff = (p.scorr_dq_coeff > 0.0f) ? p.scorr_dq_coeff : 0.3f;
+  const float dq = dq_coeff * h;
+  const float wdq = scorr_enabled ? host_poly6(dq * dq, h) : 0.0f;
+  const float inv_wdq = (wdq > 1e-12f) ? (1.0f / wdq) : 0.0f;
+  k.scorr_on = (scorr_enabled && inv_wdq > 0.0f) ? 1 : 0;
+  k.scorr_inv_wdq = inv_wdq;
+  k.scorr_neg_k = -p.scorr_k;
+  k.scorr_n = p.scorr_n;
+  k.visc_c = p.visc_c;
+  k.vort_eps = p.vort_epsilon;
+  k.vort_norm_eps = p.vort_norm_eps;
+  k.record_diag = c.record_diag ? 1 : 0;
+  k.xcd_remap = c.xcd_remap;
+  c.consts = k;
+  c.iterations = p.solver_iterations;
+  c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423
+  c.vort = p.enable_vorticity && p.vort_epsilon != 0.0f;  // core.cpp:468
+  c.restitution_any = false;
+  for (const auto& q : c.params)
+    if ((q.plane_restitution > 0.0f || q.plane_friction > 0.0f) && q.n_planes > 0)
+      c.restitution_any = true;
+}
+
+std::vector<fluidhip::ScenePlanes> scene_tables(const Ctx& c) {
+  std::vector<fluidhip::ScenePlanes> t(c.n_scenes);
+  for (int s = 0; s < c.n_scenes; ++s) {
+    const fluid_hip_params& p = c.params[s];
+    fluidhip::ScenePlanes& q = t[s];
+    std::memset(&q, 0, sizeof(q));
+    q.gx = p.gravity[0];
+    q.gy = p.gravity[1];
+    q.gz = p.gravity[2];
+    q.n_planes = p.n_planes;
+    q.restitution = p.plane_restitution;
+    q.friction = p.plane_friction;
+    q.restitution_on = (p.plane_restitution > 0.0f || p.plane_friction > 0.0f) ? 1 : 0;
+    for (int i = 0; i < p.n_planes; ++i)
+      q.planes[i] = make_float4(p.planes[i][0], p.planes[i][1], p.planes[i][2], p.planes[i][3]);
+  }
+  return t;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+void drop_graph(Ctx& c) {
+  if (c.graph_exec) hipGraphExecDestroy(c.graph_exec);
+  if (c.graph) hipGraphDestroy(c.graph);
+  c.graph_exec = nullptr;
+  c.graph = nullptr;
+}
+
+void free_all(Ctx& c) {
+  drop_graph(c);
+  for (float4** p : {&c.posA, &c.velA, &c.posB, &c.velB, &c.pred[0], &c.pred[1], &c.omega})
+    dfree(*p);
+  dfree(c.rho);
+  dfree(c.cellc);
+  for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
+                  &c.total_cells, &c.scan_partial, &c.diag_key, &c.nbr, &c.seg,
+                  &c.steps_started})
+    dfree(*p);
+  dfree(c.diag_cell);
+  dfree(c.grid);
+  dfree(c.nrange);
+  dfree(c.nbr_total);
+  dfree(c.need_cells);
+  dfree(c.planes);
+  dfree(c.flags);
+  dfree(c.stage);
+  for (auto& q : c.prof.pending) {
+    hipEventDestroy(q.a);
+    hipEventDestroy(q.b);
+  }
+  for (auto e : c.prof.pool) hipEventDestroy(e);
+  c.prof.pending.clear();
+  c.prof.pool.clear();
+}
+
+int alloc_cells(Ctx& c, long long cap) {
+  dfree(c.count);
+  dfree(c.start);
+  HIP_TRY(dalloc(&c.count, cap), "hipMalloc(cell counts)");
+  HIP_TRY(dalloc(&c.start, cap + 1), "hipMalloc(cell starts)");
+  HIP_TRY(hipMemsetAsync(c.count, 0, sizeof(int) * cap, c.stream), "hipMemset(cell counts)");
+  c.cell_cap = cap;
+  return FLUID_HIP_OK;
+}
+
+int alloc_neighbors(Ctx& c, long long cap) {
+  dfree(c.nbr);
+  HIP_TRY(dalloc(&c.nbr, cap), "hipMalloc(neighbor list)");
+  c.nbr_cap = cap;
+  return FLUID_HIP_OK;
+}
+
+int collect_profile(Ctx& c) {
+  for (auto& q : c.prof.pending) {
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, q.a, q.b);
+    if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+    auto& slot = c.prof.acc[q.name];
+    slot.first += ms;
+    slot.second += 1;
+    c.prof.pool.push_back(q.a);
+    c.prof.pool.push_back(q.b);
+  }
+  c.prof.pending.clear();
+  return FLUID_HIP_OK;
+}
+
+int queue_steps(Ctx& c, int nsteps, bool use_graph) {
+  if (use_graph && c.n > 0) {
+    if (!c.graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+      const bool prof = c.prof.on;
+      c.prof.on = false;
+      hipError_t e = fluidhip::enqueue_substep(c);
+      c.prof.on = prof;
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(c.stream, &g);
+      if (e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return hip_fail(e, "substep capture");
+      }
+      if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+      c.graph = g;
+      HIP_TRY(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0),
+              "hipGraphInstantiate");
+    }
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph_exec, c.stream), "hipGraphLaunch");
+  } else {
+    for (int s = 0; s < nsteps; ++s) {
+      hipError_t e = fluidhip::enqueue_substep(c);
+      if (e != hipSuccess) return hip_fail(e, "substep launch");
+    }
+  }
+  return FLUID_HIP_OK;
+}
+
+struct DevStatus {
+  unsigned flags;
+  int steps_started;
+  unsigned long long nbr_total;
+  unsigned long long need_cells;
+};
+
+int read_status(Ctx& c, DevStatus& st) {
+  HIP_TRY(hipMemcpyAsync(&st.flags, c.flags, sizeof(unsigned), hipMemcpyDeviceToHost, c.stream),
+          "read flags");
+  HIP_TRY(hipMemcpyAsync(&st.steps_started, c.steps_started, sizeof(int), hipMemcpyDeviceToHost,
+                         c.stream),
+          "read step counter");
+  HIP_TRY(hipMemcpyAsync(&st.nbr_total, c.nbr_total, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream),
+          "read neighbor total");
+  HIP_TRY(hipMemcpyAsync(&st.need_cells, c.need_cells, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c.stream),
+          "read cell need");
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  return collect_profile(c);
+}
+
+int check_ctx(fluid_hip_ctx* ctx) {
+  if (!ctx) return fail(FLUID_HIP_EINVAL, "context is null");
+  HIP_TRY(hipSetDevice(ctx->c.device), "hipSetDevice");
+  return FLUID_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fluid_hip_version(void) { return FLUID_HIP_ABI_VERSION; }
+
+const char* fluid_hip_last_error(void) { return g_last_error.c_str(); }
+
+int fluid_hip_device_available(int* count, const char** error) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (count) *count = 0;
+  if (error) *error = nullptr;
+  if (e == hipErrorNoDevice) return 0;
+  if (e != hipSuccess) {
+    if (error) *error = hipGetErrorString(e);
+    return 0;
+  }
+  if (count) *count = n;
+  return n > 0 ? 1 : 0;
+}
+
+int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene_counts,
+                     const fluid_hip_params* params, const fluid_hip_config* config) {
+  g_last_error.clear();
+  if (!out) return fail(FLUID_HIP_EINVAL, "out is null");
+  *out = nullptr;
+  if (n_scenes < 1 || !scene_counts) return fail(FLUID_HIP_EINVAL, "need n_scenes >= 1 and counts");
+  long long total = 0;
+  for (int s = 0; s < n_scenes; ++s) {
+    if (scene_counts[s] < 0) return fail(FLUID_HIP_EINVAL, "negative scene count");
+    total += scene_counts[s];
+  }
+  if (total > (1ll << 29)) return fail(FLUID_HIP_EINVAL, "more than 2^29 particles");
+  int rc = validate(n_scenes, params);
+  if (rc) return rc;
+  int ndev = 0;
+  if (!fluid_hip_device_available(&ndev, nullptr))
+    return fail(FLUID_HIP_ENODEV, "no HIP device available");
+  fluid_hip_config cfg{};
+  if (config) cfg = *config;
+  if (cfg.device < 0 || cfg.device >= ndev) return fail(FLUID_HIP_EINVAL, "bad device ordinal");
+  const int bs = cfg.block_size ? cfg.block_size : 256;
+  if (bs < 64 || bs > 1024 || (bs % 64) != 0)
+    return fail(FLUID_HIP_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
+
+  auto* h = new (std::nothrow) fluid_hip_ctx();
+  if (!h) return fail(FLUID_HIP_EINVAL, "out of host memory");
+  Ctx& c = h->c;
+  c.device = cfg.device;
+  c.block = bs;
+  c.xcd_remap = cfg.xcd_remap ? 1 : 0;
+  c.record_diag = cfg.record_diag != 0;
+  c.n = static_cast<int>(total);
+  c.n_scenes = n_scenes;
+  c.params.assign(params, params + n_scenes);
+  c.seg_h.resize(n_scenes + 1);
+  c.seg_h[0] = 0;
+  for (int s = 0; s < n_scenes; ++s) c.seg_h[s + 1] = c.seg_h[s] + static_cast<int>(scene_counts[s]);
+  compute_consts(c);
+
+  auto bail = [&](int code) {
+    free_all(c);
+    if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
+    delete h;
+    return code;
+  };
+  hipError_t e = hipSetDevice(c.device);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipSetDevice"));
+  e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+  c.own_stream = true;
+
+  const size_t n = static_cast<size_t>(c.n);
+  const bool need_omega = c.vort;
+#define ALLOC(ptr, cnt)                                        \
+  do {                                                         \
+    e = dalloc(&(ptr), (cnt));                                 \
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMalloc " #ptr)); \
+  } while (0)
+  ALLOC(c.posA, n);
+  ALLOC(c.velA, n);
+  ALLOC(c.posB, n);
+  ALLOC(c.velB, n);
+  ALLOC(c.pred[0], n);
+  ALLOC(c.pred[1], n);
+  if (need_omega) ALLOC(c.omega, n);
+  ALLOC(c.rho, n);
+  ALLOC(c.cellc, n);
+  ALLOC(c.key, n);
+  ALLOC(c.rank, n);
+  ALLOC(c.perm, n);
+  ALLOC(c.sid, n);
+  ALLOC(c.skey, n);
+  if (c.record_diag) {
+    ALLOC(c.diag_cell, n);
+    ALLOC(c.diag_key, n);
+  }
+  ALLOC(c.nrange, n);
+  ALLOC(c.bbox, 6 * static_cast<size_t>(n_scenes));
+  ALLOC(c.grid, static_cast<size_t>(n_scenes));
+  ALLOC(c.total_cells, 1);
+  ALLOC(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
+  ALLOC(c.nbr_total, 1);
+  ALLOC(c.need_cells, 1);
+  ALLOC(c.seg, static_cast<size_t>(n_scenes) + 1);
+  ALLOC(c.planes, static_cast<size_t>(n_scenes));
+  ALLOC(c.flags, 1);
+  ALLOC(c.steps_started, 1);
+  ALLOC(c.stage, 6 * n);
+#undef ALLOC
+  const long long cell_cap =
+      cfg.cell_capacity > 0 ? cfg.cell_capacity : std::max<long long>(1ll << 22, 8ll * c.n);
+  const long long nbr_cap =
+      cfg.neighbor_capacity > 0 ? cfg.neighbor_capacity : std::max<long long>(1ll << 16, 40ll * c.n);
+  if (cell_cap >= INT_MAX || nbr_cap >= INT_MAX) return bail(fail(FLUID_HIP_EINVAL, "capacity >= 2^31"));
+  if ((rc = alloc_cells(c, cell_cap))) return bail(rc);
+  if ((rc = alloc_neighbors(c, nbr_cap))) return bail(rc);
+  e = hipMemcpyAsync(c.seg, c.seg_h.data(), sizeof(int) * (n_scenes + 1), hipMemcpyHostToDevice,
+                     c.stream);
+  if (e != hipSuccess) return bail(hip_fail(e, "copy segments"));
+  auto tables = scene_tables(c);
+  e = hipMemcpyAsync(c.planes, tables.data(), sizeof(fluidhip::ScenePlanes) * n_scenes,
+                     hipMemcpyHostToDevice, c.stream);
+  if (e != hipSuccess) return bail(hip_fail(e, "copy scene tables"));
+  e = fluidhip::enqueue_init_grid(c);
+  if (e != hipSuccess) return bail(hip_fail(e, "init grid"));
+  e = fluidhip::enqueue_reset_flags(c);
+  if (e != hipSuccess) return bail(hip_fail(e, "reset flags"));
+  e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
+  *out = h;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_destroy(fluid_hip_ctx* ctx) {
+  if (!ctx) return FLUID_HIP_OK;
+  hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) hipStreamSynchronize(ctx->c.stream);
+  free_all(ctx->c);
+  if (ctx->c.own_stream && ctx->c.stream) hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_set_params(fluid_hip_ctx* ctx, const fluid_hip_params* params) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if ((rc = validate(c.n_scenes, params))) return rc;
+  const bool had_omega = c.omega != nullptr;
+  c.params.assign(params, params + c.n_scenes);
+  compute_consts(c);
+  if (c.vort && !had_omega) HIP_TRY(dalloc(&c.omega, std::max(c.n, 1)), "hipMalloc omega");
+  auto tables = scene_tables(c);
+  HIP_TRY(hipMemcpyAsync(c.planes, tables.data(), sizeof(fluidhip::ScenePlanes) * c.n_scenes,
+                         hipMemcpyHostToDevice, c.stream),
+          "copy scene tables");
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  drop_graph(c);
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_set_stream(fluid_hip_ctx* ctx, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if (c.own_stream) hipStreamDestroy(c.stream);
+  if (stream) {
+    c.stream = static_cast<hipStream_t>(stream);
+    c.own_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+    c.own_stream = true;
+  }
+  drop_graph(c);
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_upload(fluid_hip_ctx* ctx, const float* pos, const float* vel, float time) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (c.n > 0 && (!pos || !vel)) return fail(FLUID_HIP_EINVAL, "pos/vel is null");
+  const size_t bytes = sizeof(float) * 3 * static_cast<size_t>(c.n);
+  if (c.n > 0) {
+    HIP_TRY(hipMemcpyAsync(c.stage, pos, bytes, hipMemcpyHostToDevice, c.stream), "upload pos");
+    HIP_TRY(hipMemcpyAsync(c.stage + 3 * static_cast<size_t>(c.n), vel, bytes,
+                           hipMemcpyHostToDevice, c.stream),
+            "upload vel");
+    HIP_TRY(fluidhip::enqueue_upload(c, c.stage, c.stage + 3 * static_cast<size_t>(c.n)),
+            "upload kernel");
+  }
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  c.time = time;
+  c.uploaded = true;
+  c.stepped = false;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_download(fluid_hip_ctx* ctx, float* pos, float* vel, float* time) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  const size_t bytes = sizeof(float) * 3 * static_cast<size_t>(c.n);
+  if (c.n > 0) {
+    float* dp = c.stage;
+    float* dv = c.stage + 3 * static_cast<size_t>(c.n);
+    HIP_TRY(fluidhip::enqueue_download(c, dp, dv), "download kernel");
+    if (pos) HIP_TRY(hipMemcpyAsync(pos, dp, bytes, hipMemcpyDeviceToHost, c.stream), "download pos");
+    if (vel) HIP_TRY(hipMemcpyAsync(vel, dv, bytes, hipMemcpyDeviceToHost, c.stream), "download vel");
+  }
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if (time) *time = c.time;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_step_async(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  if (nsteps < 0) return fail(FLUID_HIP_EINVAL, "nsteps < 0");
+  if ((rc = queue_steps(c, nsteps, use_graph != 0))) return rc;
+  for (int s = 0; s < nsteps; ++s) c.time += c.params[0].dt;
+  if (nsteps > 0) c.stepped = true;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_sync(fluid_hip_ctx* ctx) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  DevStatus st{};
+  if ((rc = read_status(c, st))) return rc;
+  if (st.flags & fluidhip::kFlagRange)
+    return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
+  if (st.flags & (fluidhip::kFlagCells | fluidhip::kFlagNeighbors))
+    return fail(FLUID_HIP_ECAPACITY, "grid or neighbor capacity exceeded");
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  if (nsteps < 0) return fail(FLUID_HIP_EINVAL, "nsteps < 0");
+  int remaining = nsteps;
+  for (int attempt = 0; remaining > 0 && attempt < 16; ++attempt) {
+    HIP_TRY(fluidhip::enqueue_reset_flags(c), "reset flags");
+    if ((rc = queue_steps(c, remaining, use_graph != 0))) return rc;
+    DevStatus st{};
+    if ((rc = read_status(c, st))) return rc;
+    int done = remaining;
+    if (st.flags & fluidhip::kFlagFatal)
+      done = st.steps_started - ((st.flags & fluidhip::kFlagNeighbors) ? 1 : 0);
+    if (c.n == 0) done = remaining;
+    for (int s = 0; s < done; ++s) c.time += c.params[0].dt;  // core.cpp:614
+    if (done > 0) c.stepped = true;
+    remaining -= done;
+    if (!(st.flags & fluidhip::kFlagFatal)) break;
+    if (st.flags & fluidhip::kFlagRange)
+      return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
+    // Capacity: grow to the reported need and resume from the first step that
+    // did not complete (the state of a failed step is left untouched).
+    if (st.flags & fluidhip::kFlagCells) {
+      const long long need = static_cast<long long>(st.need_cells);
+      const long long cap = need + need / 4 + 4096;
+      if (cap >= INT_MAX) return fail(FLUID_HIP_ECAPACITY, "cell table would exceed 2^31 cells");
+      if ((rc = alloc_cells(c, cap))) return rc;
+    } else {
+      const long long need = static_cast<long long>(st.nbr_total);
+      const long long cap = need + need / 4 + 4096;
+      if (cap >= INT_MAX) return fail(FLUID_HIP_ECAPACITY, "neighbor list would exceed 2^31 entries");
+      if ((rc = alloc_neighbors(c, cap))) return rc;
+    }
+    drop_graph(c);
+  }
+  if (remaining > 0) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
+                       int32_t* nbr_count, float* rho, float* lambda) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.stepped) return fail(FLUID_HIP_ESTATE, "no step taken since upload");
+  if (cell && !c.record_diag) return fail(FLUID_HIP_ESTATE, "cells need record_diag=1");
+  const size_t n = static_cast<size_t>(c.n);
+  std::vector<float4> pos(n);
+  HIP_TRY(hipMemcpy(pos.data(), c.posA, sizeof(float4) * n, hipMemcpyDeviceToHost), "read pos");
+  std::vector<int> ids(n);
+  for (size_t s = 0; s < n; ++s) {
+    int id;
+    std::memcpy(&id, &pos[s].w, sizeof(int));
+    ids[s] = id;
+  }
+  if (sorted_ids) std::copy(ids.begin(), ids.end(), sorted_ids);
+  if (cell) {
+    std::vector<uint64_t> dc(n);
+    HIP_TRY(hipMemcpy(dc.data(), c.diag_cell, sizeof(uint64_t) * n, hipMemcpyDeviceToHost),
+            "read cells");
+    const uint64_t m = (1ull << 21) - 1ull;
+    for (size_t s = 0; s < n; ++s) {
+      const int id = ids[s];
+      cell[3 * id + 0] = static_cast<int>((dc[s] >> 42) & m) - fluidhip::kCoordLimit;
+      cell[3 * id + 1] = static_cast<int>((dc[s] >> 21) & m) - fluidhip::kCoordLimit;
+      cell[3 * id + 2] = static_cast<int>(dc[s] & m) - fluidhip::kCoordLimit;
+    }
+  }
+  if (nbr_count) {
+    std::vector<int2> nr(n);
+    HIP_TRY(hipMemcpy(nr.data(), c.nrange, sizeof(int2) * n, hipMemcpyDeviceToHost), "read ranges");
+    for (size_t s = 0; s < n; ++s) nbr_count[ids[s]] = nr[s].y;
+  }
+  if (rho) {
+    std::vector<float> r(n);
+    HIP_TRY(hipMemcpy(r.data(), c.rho, sizeof(float) * n, hipMemcpyDeviceToHost), "read rho");
+    for (size_t s = 0; s < n; ++s) rho[ids[s]] = c.iterations > 0 ? r[s] : 0.0f;
+  }
+  if (lambda) {
+    if (c.iterations > 0) {
+      std::vector<float4> P(n);
+      HIP_TRY(hipMemcpy(P.data(), c.pred[(c.iterations - 1) & 1], sizeof(float4) * n,
+                        hipMemcpyDeviceToHost),
+              "read lambda");
+      for (size_t s = 0; s < n; ++s) lambda[ids[s]] = P[s].w;
+    } else {
+      for (size_t s = 0; s < n; ++s) lambda[s] = 0.0f;
+    }
+  }
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64_t* total) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.stepped) return fail(FLUID_HIP_ESTATE, "no step taken since upload");
+  const size_t n = static_cast<size_t>(c.n);
+  std::vector<float4> pos(n);
+  std::vector<int2> nr(n);
+  HIP_TRY(hipMemcpy(pos.data(), c.posA, sizeof(float4) * n, hipMemcpyDeviceToHost), "read pos");
+  HIP_TRY(hipMemcpy(nr.data(), c.nrange, sizeof(int2) * n, hipMemcpyDeviceToHost), "read ranges");
+  unsigned long long used = 0;
+  HIP_TRY(hipMemcpy(&used, c.nbr_total, sizeof(used), hipMemcpyDeviceToHost), "read total");
+  used = std::min<unsigned long long>(used, static_cast<unsigned long long>(c.nbr_cap));
+  std::vector<int> list(used);
+  if (used)
+    HIP_TRY(hipMemcpy(list.data(), c.nbr, sizeof(int) * used, hipMemcpyDeviceToHost), "read list");
+  std::vector<int> ids(n), slot_of(n);
+  for (size_t s = 0; s < n; ++s) {
+    int id;
+    std::memcpy(&id, &pos[s].w, sizeof(int));
+    ids[s] = id;
+    slot_of[id] = static_cast<int>(s);
+  }
+  int64_t k = 0;
+  for (size_t id = 0; id < n; ++id) {
+    const int2 r = nr[slot_of[id]];
+    for (int j = 0; j < r.y; ++j) {
+      if (out && k < cap) out[k] = ids[list[r.x + j]];
+      ++k;
+    }
+  }
+  if (total) *total = k;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.stepped) return fail(FLUID_HIP_ESTATE, "no step taken since upload");
+  if (slot_keys) {
+    if (!c.record_diag) return fail(FLUID_HIP_ESTATE, "slot keys need record_diag=1");
+    HIP_TRY(hipMemcpy(slot_keys, c.diag_key, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read keys");
+  }
+  if (desc) {
+    static_assert(sizeof(fluidhip::GridDesc) == 8 * sizeof(int32_t), "GridDesc layout");
+    HIP_TRY(hipMemcpy(desc, c.grid, sizeof(fluidhip::GridDesc) * c.n_scenes, hipMemcpyDeviceToHost),
+            "read grid");
+  }
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_profile(fluid_hip_ctx* ctx, int32_t enable) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if ((rc = collect_profile(c))) return rc;
+  c.prof.on = enable != 0;
+  if (enable) c.prof.acc.clear();
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total_ms,
+                           int64_t* launches) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if ((rc = collect_profile(c))) return rc;
+  if (!kernel) return fail(FLUID_HIP_EINVAL, "kernel name is null");
+  auto it = c.prof.acc.find(kernel);
+  if (total_ms) *total_ms = it == c.prof.acc.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == c.prof.acc.end() ? 0 : it->second.second;
+  return FLUID_HIP_OK;
+}
+
+}  // extern "C"

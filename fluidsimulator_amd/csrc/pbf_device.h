@@ -1,0 +1,164 @@
+// Device-side data structures and helpers of the gfx950 PBF step.
+//
+// Arithmetic in this file follows the reference CPU step
+// (core/src/core.cpp:35-71, :119-615) operation for operation; the whole
+// library is compiled with -ffp-contract=off and IEEE division/sqrt, so each
+// expression rounds exactly as the reference's does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fluidhip {
+
+constexpr int kMaxPlanes = 8;
+constexpr int kBrick = 4;        // cells per brick edge
+constexpr int kBrickCells = 64;  // kBrick^3
+constexpr int kCoordLimit = 1 << 20;
+
+// Device error flags (sticky until the host clears them).
+constexpr unsigned kFlagRange = 1u;     // non-finite / out-of-range position
+constexpr unsigned kFlagCells = 2u;     // cell table capacity exceeded
+constexpr unsigned kFlagNeighbors = 4u; // neighbor list capacity exceeded
+constexpr unsigned kFlagFatal = kFlagRange | kFlagCells | kFlagNeighbors;
+
+// Loop-invariant constants of one substep, computed on the host with the
+// same float expressions the reference evaluates inside its loops.
+struct StepConsts {
+  int n;            // particles, all scenes
+  int n_scenes;
+  float dt;
+  float h;
+  float inv_h;      // 1.0f / h            (core.cpp:29)
+  float h2;         // h * h               (core.cpp:140)
+  float min_r2;     // (0.01f*h)^2         (core.cpp:141-142)
+  float w_coeff;    // 315 / (64 pi h^9)   (core.cpp:44)
+  float w0;         // poly6(0, h)         (core.cpp:319)
+  float s_coeff;    // -45 / (pi h^6)      (core.cpp:55)
+  float inv_density;
+  float mass;
+  float grad_scale; // m * (1/rho0)        (core.cpp:273)
+  float eps;
+  int scorr_on;     // enable_scorr && k != 0 && inv_wdq > 0
+  float scorr_inv_wdq;
+  float scorr_neg_k;  // -k
+  int scorr_n;
+  float visc_c;
+  float vort_eps;
+  float vort_norm_eps;
+  int record_diag;
+  int xcd_remap;
+};
+
+struct ScenePlanes {
+  float gx, gy, gz;
+  int n_planes;
+  float restitution;   // plane_restitution
+  float friction;      // plane_friction
+  int restitution_on;  // e > 0 || f > 0 (core.cpp:573)
+  int pad;
+  float4 planes[kMaxPlanes];  // (nx, ny, nz, d)
+};
+
+// Per-scene grid descriptor, rebuilt every substep from the predicted
+// positions' bounding box.
+struct GridDesc {
+  int ox, oy, oz;     // cell coordinate of table origin (bbox min - 1)
+  int nbx, nby, nbz;  // bricks per axis
+  int cell_off;       // first key of this scene
+  int ncells;         // nbx*nby*nbz*64
+};
+
+__device__ __forceinline__ float poly6(float r2, const StepConsts& c) {
+  // core.cpp:35-46 with the coefficient hoisted (same float value).
+  if (r2 > c.h2) return 0.0f;
+  const float diff = c.h2 - r2;
+  const float diff3 = diff * diff * diff;
+  return c.w_coeff * diff3;
+}
+
+__device__ __forceinline__ float spiky(float r, const StepConsts& c) {
+  // core.cpp:48-57 with the coefficient hoisted.
+  if (r > c.h) return 0.0f;
+  const float diff = c.h - r;
+  return c.s_coeff * diff * diff;
+}
+
+__device__ __forceinline__ float pow_ratio(float ratio, int n) {
+  // core.cpp:59-71.  n outside {2,3,4} goes through powf, whose last bit
+  // may differ from the host libm's.
+  if (n == 2) return ratio * ratio;
+  if (n == 3) return ratio * ratio * ratio;
+  if (n == 4) {
+    const float r2 = ratio * ratio;
+    return r2 * r2;
+  }
+  return powf(ratio, static_cast<float>(n));
+}
+
+__device__ __forceinline__ int morton_in_brick(int x, int y, int z) {
+  // 2-bit x, y, z interleaved: x0 y0 z0 x1 y1 z1
+  return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) | ((y & 2) << 3) |
+         ((z & 2) << 4);
+}
+
+// Key of the cell at table-relative coordinates (rx, ry, rz) >= 0.
+__device__ __forceinline__ int cell_key(int rx, int ry, int rz, const GridDesc& g) {
+  const int b = ((rz >> 2) * g.nby + (ry >> 2)) * g.nbx + (rx >> 2);
+  return g.cell_off + b * kBrickCells + morton_in_brick(rx & 3, ry & 3, rz & 3);
+}
+
+__device__ __forceinline__ int floor_cell(float v, float inv_h, bool& bad) {
+  // static_cast<int>(std::floor(v * inv)) — core.cpp:28-34
+  const float s = v * inv_h;
+  if (!(fabsf(s) < static_cast<float>(kCoordLimit))) {
+    bad = true;
+    return 0;
+  }
+  return static_cast<int>(floorf(s));
+}
+
+__device__ __forceinline__ uint64_t pack_cell(int x, int y, int z) {
+  const uint64_t m = (1ull << 21) - 1ull;
+  return ((uint64_t(x + kCoordLimit) & m) << 42) | ((uint64_t(y + kCoordLimit) & m) << 21) |
+         (uint64_t(z + kCoordLimit) & m);
+}
+
+__device__ __forceinline__ void unpack_cell(uint64_t k, int& x, int& y, int& z) {
+  const uint64_t m = (1ull << 21) - 1ull;
+  x = int((k >> 42) & m) - kCoordLimit;
+  y = int((k >> 21) & m) - kCoordLimit;
+  z = int(k & m) - kCoordLimit;
+}
+
+// Scene of a slot: scenes are contiguous, fixed-size slot segments.
+__device__ __forceinline__ int scene_of(int slot, const int* __restrict__ seg, int n_scenes) {
+  if (n_scenes == 1) return 0;
+  int lo = 0, hi = n_scenes - 1;  // seg has n_scenes+1 entries
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg[mid] <= slot)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Workgroup -> particle tile, so that each XCD (dispatched round-robin)
+// streams a contiguous range of the sorted arrays through its own L2.
+__device__ __forceinline__ int tile_index(int remap) {
+  const int bid = blockIdx.x;
+  if (!remap) return bid;
+  const int nb = gridDim.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = bid & 7, local = bid >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+__device__ __forceinline__ float3 ld3(const float4* p, int i) {
+  const float* f = reinterpret_cast<const float*>(p + i);
+  return make_float3(f[0], f[1], f[2]);
+}
+
+}  // namespace fluidhip

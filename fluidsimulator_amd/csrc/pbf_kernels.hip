@@ -1,0 +1,929 @@
+// gfx950 kernels of one PBF substep and the plan that queues them.
+//
+// Functional spec: the reference CPU step `fluid::step`
+// (core/src/core.cpp:119-615).  Everything here is written for CDNA4:
+// 64-lane wavefronts, float4 AoS state resident in HBM, a dense per-scene
+// cell table of 4x4x4-cell bricks (Z-order inside a brick), an atomic
+// counting sort followed by a by-id rank inside each cell, and CSR neighbor
+// lists allocated per workgroup.
+//
+// Phase order of one substep (one launch each; every launch covers all
+// scenes of a batch):
+//   predict_bbox  v' = v + g dt, pred0 = x + v' dt, cell = floor(pred0/h),
+//                 per-scene bounding box of cells          (core.cpp:150-161)
+//   grid_setup    per-scene table origin/extent, key offsets, counters
+//   cell_count    key of each particle, atomic cell histogram
+//   scan x3       exclusive scan of the histogram -> cell starts
+//   scatter       slot = start[key] + rank
+//   reorder       rank by original id inside each cell (the reference's
+//                 (cell, id) order, core.cpp:173-183), gather state into
+//                 sorted order and recompute pred0 there
+//   neighbors     27 cells in (dz, dy, dx) order, each by ascending id,
+//                 j != i and r^2 < h^2 on pred0       (core.cpp:205-247)
+//   lambda/delta  x solver_iterations, Jacobi double buffer (core.cpp:277-408);
+//                 the last delta also commits x and v  (core.cpp:410-421)
+//   xsph          (core.cpp:423-466)
+//   vorticity     omega, then eta + confinement        (core.cpp:468-571)
+//   plane restitution/friction fused into whichever kernel ends the chain
+//                                                      (core.cpp:573-612)
+
+#include <climits>
+
+#include "pbf_ctx.h"
+#include "pbf_device.h"
+
+namespace fluidhip {
+
+namespace {
+
+constexpr int kWave = 64;
+
+// Kernels launched after the grid setup stop as soon as an earlier phase has
+// raised a fatal flag; the committed state is then left untouched.
+__device__ __forceinline__ bool fatal(const unsigned* flags) {
+  return (*flags & kFlagFatal) != 0u;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(v, o, kWave);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix sum over the workgroup; *total receives the sum.
+// `lds` needs blockDim.x / 64 ints.
+__device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  const int inc = wave_incl_scan(v);
+  if (lane == kWave - 1) lds[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < nw; ++i) {
+      const int t = lds[i];
+      lds[i] = run;
+      run += t;
+    }
+    lds[nw] = run;
+  }
+  __syncthreads();
+  const int r = lds[w] + inc - v;
+  *total = lds[nw];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void project_planes(float& qx, float& qy, float& qz,
+                                               const ScenePlanes& sp) {
+  // core.cpp:372-389: sequential push-out along each plane normal.
+  for (int p = 0; p < sp.n_planes; ++p) {
+    const float4 pl = sp.planes[p];
+    const float sd = pl.x * qx + pl.y * qy + pl.z * qz - pl.w;
+    const float pen = -sd;
+    if (pen > 0.0f) {
+      qx += pl.x * pen;
+      qy += pl.y * pen;
+      qz += pl.z * pen;
+    }
+  }
+}
+
+__device__ __forceinline__ void restitute(float& vx, float& vy, float& vz, float px, float py,
+                                          float pz, const ScenePlanes& sp) {
+  // core.cpp:573-612
+  if (!sp.restitution_on) return;
+  for (int p = 0; p < sp.n_planes; ++p) {
+    const float4 pl = sp.planes[p];
+    const float sd = pl.x * px + pl.y * py + pl.z * pz - pl.w;
+    if (sd <= 0.0f) {
+      const float vn = pl.x * vx + pl.y * vy + pl.z * vz;
+      float vn_new = vn;
+      if (vn < 0.0f) vn_new = -sp.restitution * vn;
+      const float tx = vx - vn * pl.x;
+      const float ty = vy - vn * pl.y;
+      const float tz = vz - vn * pl.z;
+      const float scale = 1.0f - sp.friction;
+      vx = tx * scale + vn_new * pl.x;
+      vy = ty * scale + vn_new * pl.y;
+      vz = tz * scale + vn_new * pl.z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- transfers
+
+__global__ void k_upload(int n, const float* __restrict__ pos, const float* __restrict__ vel,
+                         float4* __restrict__ posA, float4* __restrict__ velA) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = static_cast<size_t>(n);
+  posA[i] = make_float4(pos[i], pos[N + i], pos[2 * N + i], __int_as_float(i));
+  velA[i] = make_float4(vel[i], vel[N + i], vel[2 * N + i], 0.0f);
+}
+
+__global__ void k_download(int n, const float4* __restrict__ posA, const float4* __restrict__ velA,
+                           float* __restrict__ pos, float* __restrict__ vel) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4 p = posA[s];
+  const float4 v = velA[s];
+  const size_t id = static_cast<size_t>(__float_as_int(p.w));
+  const size_t N = static_cast<size_t>(n);
+  pos[id] = p.x;
+  pos[N + id] = p.y;
+  pos[2 * N + id] = p.z;
+  vel[id] = v.x;
+  vel[N + id] = v.y;
+  vel[2 * N + id] = v.z;
+}
+
+__global__ void k_reset_flags(unsigned* flags, int* steps_started) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *flags = 0u;
+    *steps_started = 0;
+  }
+}
+
+__global__ void k_init_bbox(int n_scenes, int* bbox) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_scenes) return;
+  bbox[6 * s + 0] = INT_MAX;
+  bbox[6 * s + 1] = INT_MAX;
+  bbox[6 * s + 2] = INT_MAX;
+  bbox[6 * s + 3] = INT_MIN;
+  bbox[6 * s + 4] = INT_MIN;
+  bbox[6 * s + 5] = INT_MIN;
+}
+
+// ---------------------------------------------------------------- grid build
+
+__global__ __launch_bounds__(1024) void k_predict_bbox(
+    StepConsts c, const float4* __restrict__ pos, const float4* __restrict__ vel,
+    const ScenePlanes* __restrict__ sp, const int* __restrict__ seg,
+    uint64_t* __restrict__ cellc, int* __restrict__ bbox, unsigned* __restrict__ flags) {
+  __shared__ int red[6][1024 / kWave];
+  const int base = tile_index(c.xcd_remap) * blockDim.x;
+  const int i = base + threadIdx.x;
+  const bool active = i < c.n;
+  int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX;
+  int mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+  bool bad = false;
+  int sc = 0;
+  if (active) {
+    sc = scene_of(i, seg, c.n_scenes);
+    const ScenePlanes& s = sp[sc];
+    const float3 x = ld3(pos, i);
+    const float3 v = ld3(vel, i);
+    const float vx = v.x + s.gx * c.dt;  // core.cpp:155-157
+    const float vy = v.y + s.gy * c.dt;
+    const float vz = v.z + s.gz * c.dt;
+    const float px = x.x + vx * c.dt;    // core.cpp:158-160
+    const float py = x.y + vy * c.dt;
+    const float pz = x.z + vz * c.dt;
+    const int cx = floor_cell(px, c.inv_h, bad);
+    const int cy = floor_cell(py, c.inv_h, bad);
+    const int cz = floor_cell(pz, c.inv_h, bad);
+    cellc[i] = pack_cell(cx, cy, cz);
+    mn0 = mx0 = cx;
+    mn1 = mx1 = cy;
+    mn2 = mx2 = cz;
+  }
+  if (bad) atomicOr(flags, kFlagRange);
+  const int last = min(base + static_cast<int>(blockDim.x), c.n) - 1;
+  if (base > last) return;  // whole block past the end (uniform)
+  const int s_first = scene_of(base, seg, c.n_scenes);
+  const int s_last = scene_of(last, seg, c.n_scenes);
+  if (s_first == s_last) {
+    // one scene: reduce the workgroup, then one set of atomics
+    mn0 = wave_min(mn0), mn1 = wave_min(mn1), mn2 = wave_min(mn2);
+    mx0 = wave_max(mx0), mx1 = wave_max(mx1), mx2 = wave_max(mx2);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+      red[0][w] = mn0, red[1][w] = mn1, red[2][w] = mn2;
+      red[3][w] = mx0, red[4][w] = mx1, red[5][w] = mx2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = blockDim.x / kWave;
+      for (int k = 1; k < nw; ++k) {
+        mn0 = min(mn0, red[0][k]), mn1 = min(mn1, red[1][k]), mn2 = min(mn2, red[2][k]);
+        mx0 = max(mx0, red[3][k]), mx1 = max(mx1, red[4][k]), mx2 = max(mx2, red[5][k]);
+      }
+      int* b = bbox + 6 * s_first;
+      atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
+      atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+    }
+  } else if (active) {
+    int* b = bbox + 6 * sc;
+    atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
+    atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+  }
+}
+
+// One workgroup: per-scene table extents and key offsets; resets the
+// per-step counters.
+__global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, int* __restrict__ bbox,
+                                                    GridDesc* __restrict__ grid,
+                                                    int* __restrict__ total_cells,
+                                                    long long cell_cap,
+                                                    unsigned* __restrict__ flags,
+                                                    unsigned long long* __restrict__ nbr_total,
+                                                    unsigned long long* __restrict__ need_cells,
+                                                    int* __restrict__ steps_started) {
+  __shared__ long long sh[257];
+  __shared__ long long carry;
+  // After a failed substep the counters keep what the host needs to size
+  // the retry; only the bounding boxes are reset.
+  const bool was_fatal = (*flags & kFlagFatal) != 0u;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < n_scenes; b0 += blockDim.x) {
+    const int s = b0 + threadIdx.x;
+    long long nc = 0;
+    GridDesc g{};
+    if (s < n_scenes) {
+      int* bb = bbox + 6 * s;
+      const long long lx = bb[0], ly = bb[1], lz = bb[2];
+      const long long hx = bb[3], hy = bb[4], hz = bb[5];
+      if (lx <= hx && ly <= hy && lz <= hz) {
+        const long long sx = hx - lx + 3, sy = hy - ly + 3, sz = hz - lz + 3;
+        const long long bx = (sx + 3) / 4, by = (sy + 3) / 4, bz = (sz + 3) / 4;
+        nc = bx * by * bz * kBrickCells;
+        g.ox = static_cast<int>(lx - 1);
+        g.oy = static_cast<int>(ly - 1);
+        g.oz = static_cast<int>(lz - 1);
+        g.nbx = static_cast<int>(bx);
+        g.nby = static_cast<int>(by);
+        g.nbz = static_cast<int>(bz);
+      }
+      bb[0] = INT_MAX, bb[1] = INT_MAX, bb[2] = INT_MAX;
+      bb[3] = INT_MIN, bb[4] = INT_MIN, bb[5] = INT_MIN;
+    }
+    // exclusive scan of nc over the workgroup
+    sh[threadIdx.x] = nc;
+    __syncthreads();
+    for (int o = 1; o < static_cast<int>(blockDim.x); o <<= 1) {
+      const long long t = (threadIdx.x >= static_cast<unsigned>(o)) ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    const long long excl = sh[threadIdx.x] - nc + carry;
+    if (s < n_scenes) {
+      g.cell_off = static_cast<int>(excl < cell_cap ? excl : 0);
+      g.ncells = static_cast<int>(nc < cell_cap ? nc : 0);
+      grid[s] = g;
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += sh[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && !was_fatal) {
+    *nbr_total = 0ull;
+    if (carry > cell_cap) {
+      atomicOr(flags, kFlagCells);
+      *total_cells = 0;
+      *need_cells = static_cast<unsigned long long>(carry);
+    } else {
+      *total_cells = static_cast<int>(carry);
+      atomicAdd(steps_started, 1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
+                                                     const uint64_t* __restrict__ cellc,
+                                                     const int* __restrict__ seg,
+                                                     const GridDesc* __restrict__ grid,
+                                                     int* __restrict__ count, int* __restrict__ key,
+                                                     int* __restrict__ rank,
+                                                     const unsigned* __restrict__ flags) {
+  const int i = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (i >= c.n || fatal(flags)) return;
+  const GridDesc g = grid[scene_of(i, seg, c.n_scenes)];
+  int x, y, z;
+  unpack_cell(cellc[i], x, y, z);
+  const int k = cell_key(x - g.ox, y - g.oy, z - g.oz, g);
+  rank[i] = atomicAdd(count + k, 1);
+  key[i] = k;
+}
+
+// ---- exclusive scan over a device-sized array (three launches, fixed grid)
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ in,
+                                                     const int* __restrict__ d_len,
+                                                     int* __restrict__ partial,
+                                                     const unsigned* __restrict__ flags) {
+  __shared__ int lds[8];
+  if (fatal(flags)) return;
+  const int len = *d_len;
+  const int G = gridDim.x;
+  const int chunk = (len + G - 1) / G;
+  const int b0 = blockIdx.x * chunk;
+  const int b1 = min(len, b0 + chunk);
+  int acc = 0;
+  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) acc += in[i];
+  int total;
+  block_excl_scan(acc, lds, &total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partial, int G,
+                                                        const unsigned* __restrict__ flags) {
+  __shared__ int lds[17];
+  if (fatal(flags)) return;
+  const int v = (static_cast<int>(threadIdx.x) < G) ? partial[threadIdx.x] : 0;
+  int total;
+  const int e = block_excl_scan(v, lds, &total);
+  if (static_cast<int>(threadIdx.x) < G) partial[threadIdx.x] = e;
+  if (threadIdx.x == 0) partial[G] = total;
+}
+
+// out[i] = exclusive prefix; out[len] = grand total.  zero_in clears `in`
+// (the cell histogram is reused by the next substep).
+__global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
+                                                   const int* __restrict__ d_len,
+                                                   const int* __restrict__ partial,
+                                                   int* __restrict__ out, int zero_in,
+                                                   const unsigned* __restrict__ flags) {
+  __shared__ int lds[8];
+  if (fatal(flags)) return;
+  const int len = *d_len;
+  const int G = gridDim.x;
+  const int chunk = (len + G - 1) / G;
+  const int b0 = blockIdx.x * chunk;
+  const int b1 = min(len, b0 + chunk);
+  int run = partial[blockIdx.x];
+  for (int t0 = b0; t0 < b1; t0 += blockDim.x) {
+    const int i = t0 + threadIdx.x;
+    const int v = (i < b1) ? in[i] : 0;
+    int total;
+    const int e = block_excl_scan(v, lds, &total);
+    if (i < b1) {
+      out[i] = run + e;
+      if (zero_in) in[i] = 0;
+    }
+    run += total;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = partial[G];
+}
+
+__global__ __launch_bounds__(1024) void k_scatter(StepConsts c, const int* __restrict__ key,
+                                                  const int* __restrict__ rank,
+                                                  const int* __restrict__ start,
+                                                  const float4* __restrict__ pos,
+                                                  int* __restrict__ perm, int* __restrict__ sid,
+                                                  int* __restrict__ skey,
+                                                  const unsigned* __restrict__ flags) {
+  const int p = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (p >= c.n || fatal(flags)) return;
+  const int k = key[p];
+  const int slot = start[k] + rank[p];
+  perm[slot] = p;
+  sid[slot] = __float_as_int(reinterpret_cast<const float*>(pos + p)[3]);
+  skey[slot] = k;
+}
+
+// Rank by original id inside the cell, then gather the particle into its
+// final slot and recompute pred0 there (same float ops as predict_bbox).
+__global__ __launch_bounds__(1024) void k_reorder(
+    StepConsts c, const int* __restrict__ perm, const int* __restrict__ sid,
+    const int* __restrict__ skey, const int* __restrict__ start, const int* __restrict__ seg,
+    const ScenePlanes* __restrict__ sp, const float4* __restrict__ posA,
+    const float4* __restrict__ velA, float4* __restrict__ posB, float4* __restrict__ velB,
+    float4* __restrict__ pred, const uint64_t* __restrict__ cellc,
+    uint64_t* __restrict__ diag_cell, int* __restrict__ diag_key,
+    const unsigned* __restrict__ flags) {
+  const int u = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (u >= c.n || fatal(flags)) return;
+  const int p = perm[u];
+  const int id = sid[u];
+  const int k = skey[u];
+  const int a = start[k];
+  const int b = start[k + 1];
+  int r = 0;
+  for (int t = a; t < b; ++t) r += (sid[t] < id) ? 1 : 0;
+  const int dest = a + r;
+  const ScenePlanes& s = sp[scene_of(u, seg, c.n_scenes)];
+  const float4 x = posA[p];
+  const float4 v = velA[p];
+  const float vx = v.x + s.gx * c.dt;
+  const float vy = v.y + s.gy * c.dt;
+  const float vz = v.z + s.gz * c.dt;
+  posB[dest] = x;
+  velB[dest] = make_float4(vx, vy, vz, 0.0f);
+  pred[dest] = make_float4(x.x + vx * c.dt, x.y + vy * c.dt, x.z + vz * c.dt, 0.0f);
+  if (c.record_diag) {
+    diag_cell[dest] = cellc[p];
+    diag_key[dest] = k;
+  }
+}
+
+// ---------------------------------------------------------------- neighbors
+
+template <bool kFill>
+__device__ __forceinline__ int walk_cells(int s, float px, float py, float pz, int rx, int ry,
+                                          int rz, const GridDesc& g, const StepConsts& c,
+                                          const int* __restrict__ start,
+                                          const float4* __restrict__ pred, int* __restrict__ out) {
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int k = cell_key(rx + dx, ry + dy, rz + dz, g);
+        const int a = start[k];
+        const int b = start[k + 1];
+        for (int t = a; t < b; ++t) {
+          if (t == s) continue;
+          const float3 q = ld3(pred, t);
+          const float ex = px - q.x;
+          const float ey = py - q.y;
+          const float ez = pz - q.z;
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          if (r2 < c.h2) {
+            if (kFill) out[cnt] = t;
+            ++cnt;
+          }
+        }
+      }
+  return cnt;
+}
+
+__global__ __launch_bounds__(1024) void k_neighbors(
+    StepConsts c, const float4* __restrict__ pred, const int* __restrict__ seg,
+    const GridDesc* __restrict__ grid, const int* __restrict__ start, int2* __restrict__ nrange,
+    int* __restrict__ nbr, long long nbr_cap, unsigned long long* __restrict__ nbr_total,
+    unsigned* __restrict__ flags) {
+  __shared__ int lds[1024 / kWave + 1];
+  __shared__ long long base_sh;
+  __shared__ int abort_sh;
+  // Other workgroups of this launch may raise the neighbor flag at any time:
+  // read it once per workgroup so the barrier below stays uniform.
+  if (threadIdx.x == 0) abort_sh = fatal(flags) ? 1 : 0;
+  __syncthreads();
+  if (abort_sh) return;
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  bool active = s < c.n;
+  float px = 0.f, py = 0.f, pz = 0.f;
+  int rx = 0, ry = 0, rz = 0;
+  GridDesc g{};
+  int cnt = 0;
+  if (active) {
+    const float3 p = ld3(pred, s);
+    px = p.x, py = p.y, pz = p.z;
+    g = grid[scene_of(s, seg, c.n_scenes)];
+    bool bad = false;
+    rx = floor_cell(px, c.inv_h, bad) - g.ox;
+    ry = floor_cell(py, c.inv_h, bad) - g.oy;
+    rz = floor_cell(pz, c.inv_h, bad) - g.oz;
+    // The table has a one-cell margin around the bounding box of these same
+    // cells; anything else would index outside it, so refuse instead.
+    if (bad || rx < 1 || ry < 1 || rz < 1 || rx > 4 * g.nbx - 2 || ry > 4 * g.nby - 2 ||
+        rz > 4 * g.nbz - 2) {
+      atomicOr(flags, kFlagRange);
+      active = false;
+    } else {
+      cnt = walk_cells<false>(s, px, py, pz, rx, ry, rz, g, c, start, pred, nullptr);
+    }
+  }
+  int total;
+  const int pre = block_excl_scan(cnt, lds, &total);
+  if (threadIdx.x == 0)
+    base_sh = static_cast<long long>(atomicAdd(nbr_total, static_cast<unsigned long long>(total)));
+  __syncthreads();
+  if (!active) return;
+  const long long off = base_sh + pre;
+  if (off + cnt > nbr_cap) {
+    atomicOr(flags, kFlagNeighbors);
+    nrange[s] = make_int2(0, 0);
+    return;
+  }
+  walk_cells<true>(s, px, py, pz, rx, ry, rz, g, c, start, pred, nbr + off);
+  nrange[s] = make_int2(static_cast<int>(off), cnt);
+}
+
+// ---------------------------------------------------------------- solver
+
+// lambda pass (core.cpp:281-329).  Writes lambda into pred[s].w.
+__global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restrict__ P,
+                                                 const int2* __restrict__ nrange,
+                                                 const int* __restrict__ nbr,
+                                                 float* __restrict__ rho_out, int write_rho,
+                                                 const unsigned* __restrict__ flags) {
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (s >= c.n || fatal(flags)) return;
+  const int2 nr = nrange[s];
+  const float3 p = ld3(P, s);
+  float rho = 0.0f, gsx = 0.0f, gsy = 0.0f, gsz = 0.0f, sg2 = 0.0f;
+  const float ngs = -c.grad_scale;
+  for (int j = 0; j < nr.y; ++j) {
+    const int t = nbr[nr.x + j];
+    const float3 q = ld3(P, t);
+    const float dx = p.x - q.x;
+    const float dy = p.y - q.y;
+    const float dz = p.z - q.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    rho += poly6(r2, c);
+    if (r2 < c.h2) {
+      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
+      const float f = spiky(r, c);
+      const float gx = f * dx, gy = f * dy, gz = f * dz;
+      gsx += gx;
+      gsy += gy;
+      gsz += gz;
+      const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
+      sg2 += jx * jx + jy * jy + jz * jz;
+    }
+  }
+  rho += c.w0;
+  rho *= c.mass;
+  const float C = rho * c.inv_density - 1.0f;
+  const float gix = c.grad_scale * gsx, giy = c.grad_scale * gsy, giz = c.grad_scale * gsz;
+  sg2 += gix * gix + giy * giy + giz * giz;
+  reinterpret_cast<float*>(P + s)[3] = -C / (sg2 + c.eps);
+  if (write_rho) rho_out[s] = rho;
+}
+
+// delta pass with plane projection and apply (core.cpp:334-407).  kLast
+// also commits (core.cpp:414-421) into posA/vout; kSolve=false is the
+// zero-iteration commit.  kRest fuses plane restitution (core.cpp:573-612).
+template <bool kSolve, bool kLast, bool kRest>
+__global__ __launch_bounds__(1024) void k_delta(
+    StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext,
+    const int2* __restrict__ nrange, const int* __restrict__ nbr, const int* __restrict__ seg,
+    const ScenePlanes* __restrict__ sp, const float4* __restrict__ posB,
+    float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
+    const unsigned* __restrict__ flags) {
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (s >= c.n || fatal(flags)) return;
+  const float4 pi = P[s];
+  float nx = pi.x, ny = pi.y, nz = pi.z;
+  const ScenePlanes& sc = sp[scene_of(s, seg, c.n_scenes)];
+  if (kSolve) {
+    const int2 nr = nrange[s];
+    float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
+    for (int j = 0; j < nr.y; ++j) {
+      const int t = nbr[nr.x + j];
+      const float4 q = P[t];
+      const float dx = pi.x - q.x;
+      const float dy = pi.y - q.y;
+      const float dz = pi.z - q.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < c.h2) {
+        const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
+        const float f = spiky(r, c);
+        float sv = pi.w + q.w;
+        if (c.scorr_on) {
+          const float W = poly6(r2, c);
+          const float ratio = W * c.scorr_inv_wdq;
+          const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
+          sv += scorr;
+        }
+        ddx += sv * f * dx;
+        ddy += sv * f * dy;
+        ddz += sv * f * dz;
+      }
+    }
+    ddx *= c.inv_density;
+    ddy *= c.inv_density;
+    ddz *= c.inv_density;
+    if (sc.n_planes > 0) {
+      float qx = pi.x + ddx, qy = pi.y + ddy, qz = pi.z + ddz;
+      project_planes(qx, qy, qz, sc);
+      ddx = qx - pi.x;
+      ddy = qy - pi.y;
+      ddz = qz - pi.z;
+    }
+    nx = pi.x + ddx;  // core.cpp:404-406
+    ny = pi.y + ddy;
+    nz = pi.z + ddz;
+  }
+  if (!kLast) {
+    Pnext[s] = make_float4(nx, ny, nz, 0.0f);
+    return;
+  }
+  const float4 x = posB[s];
+  float vx = (nx - x.x) / c.dt;  // core.cpp:415-417
+  float vy = (ny - x.y) / c.dt;
+  float vz = (nz - x.z) / c.dt;
+  posA[s] = make_float4(nx, ny, nz, x.w);
+  if (kRest) restitute(vx, vy, vz, nx, ny, nz, sc);
+  vout[s] = make_float4(vx, vy, vz, kSolve ? rho[s] : 0.0f);
+}
+
+// XSPH (core.cpp:430-465); vin.w carries rho_j of the last lambda pass.
+template <bool kRest>
+__global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __restrict__ pos,
+                                               const float4* __restrict__ vin,
+                                               float4* __restrict__ vout,
+                                               const int2* __restrict__ nrange,
+                                               const int* __restrict__ nbr,
+                                               const int* __restrict__ seg,
+                                               const ScenePlanes* __restrict__ sp,
+                                               const unsigned* __restrict__ flags) {
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (s >= c.n || fatal(flags)) return;
+  const int2 nr = nrange[s];
+  const float3 xi = ld3(pos, s);
+  const float4 vi = vin[s];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int j = 0; j < nr.y; ++j) {
+    const int t = nbr[nr.x + j];
+    const float3 xj = ld3(pos, t);
+    const float dx = xi.x - xj.x;
+    const float dy = xi.y - xj.y;
+    const float dz = xi.z - xj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < c.h2) {
+      const float4 vj = vin[t];
+      const float W = poly6(r2, c);
+      const float irho = (vj.w > 0.0f) ? (c.mass / vj.w) : 0.0f;
+      ax += (vj.x - vi.x) * W * irho;
+      ay += (vj.y - vi.y) * W * irho;
+      az += (vj.z - vi.z) * W * irho;
+    }
+  }
+  float vx = vi.x + c.visc_c * ax;
+  float vy = vi.y + c.visc_c * ay;
+  float vz = vi.z + c.visc_c * az;
+  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, seg, c.n_scenes)]);
+  vout[s] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// vorticity: omega and |omega| (core.cpp:472-508)
+__global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ vel,
+                                                     float4* __restrict__ omega,
+                                                     const int2* __restrict__ nrange,
+                                                     const int* __restrict__ nbr,
+                                                     const unsigned* __restrict__ flags) {
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (s >= c.n || fatal(flags)) return;
+  const int2 nr = nrange[s];
+  const float3 xi = ld3(pos, s);
+  const float3 vi = ld3(vel, s);
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  for (int j = 0; j < nr.y; ++j) {
+    const int t = nbr[nr.x + j];
+    const float3 xj = ld3(pos, t);
+    const float dx = xi.x - xj.x;
+    const float dy = xi.y - xj.y;
+    const float dz = xi.z - xj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < c.h2) {
+      const float3 vj = ld3(vel, t);
+      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
+      const float f = spiky(r, c);
+      const float gx = f * dx, gy = f * dy, gz = f * dz;
+      const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+      ox += dvy * gz - dvz * gy;
+      oy += dvz * gx - dvx * gz;
+      oz += dvx * gy - dvy * gx;
+    }
+  }
+  omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
+}
+
+// vorticity: eta, confinement force, velocity update (core.cpp:512-570),
+// in place on vel (no neighbor reads vel here).
+template <bool kRest>
+__global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ omega,
+                                                     float4* __restrict__ vel,
+                                                     const int2* __restrict__ nrange,
+                                                     const int* __restrict__ nbr,
+                                                     const int* __restrict__ seg,
+                                                     const ScenePlanes* __restrict__ sp,
+                                                     const unsigned* __restrict__ flags) {
+  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (s >= c.n || fatal(flags)) return;
+  const int2 nr = nrange[s];
+  const float3 xi = ld3(pos, s);
+  const float4 om = omega[s];
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+  for (int j = 0; j < nr.y; ++j) {
+    const int t = nbr[nr.x + j];
+    const float3 xj = ld3(pos, t);
+    const float dx = xi.x - xj.x;
+    const float dy = xi.y - xj.y;
+    const float dz = xi.z - xj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < c.h2) {
+      const float omj = reinterpret_cast<const float*>(omega + t)[3];
+      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
+      const float f = spiky(r, c);
+      const float gx = f * dx, gy = f * dy, gz = f * dz;
+      const float co = omj - om.w;
+      ex += co * gx;
+      ey += co * gy;
+      ez += co * gz;
+    }
+  }
+  const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  if (len > c.vort_norm_eps) {
+    const float inv = 1.0f / len;
+    nx = ex * inv;
+    ny = ey * inv;
+    nz = ez * inv;
+  }
+  const float fx = c.vort_eps * (ny * om.z - nz * om.y);
+  const float fy = c.vort_eps * (nz * om.x - nx * om.z);
+  const float fz = c.vort_eps * (nx * om.y - ny * om.x);
+  const float4 v = vel[s];
+  float vx = v.x + c.dt * fx;
+  float vy = v.y + c.dt * fy;
+  float vz = v.z + c.dt * fz;
+  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, seg, c.n_scenes)]);
+  vel[s] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// ---------------------------------------------------------------- launching
+
+struct Launcher {
+  Ctx& c;
+  template <typename F>
+  hipError_t operator()(const char* name, F&& f) {
+    if (c.prof.on) {
+      hipEvent_t a, b;
+      if (c.prof.pool.size() >= 2) {
+        a = c.prof.pool.back();
+        c.prof.pool.pop_back();
+        b = c.prof.pool.back();
+        c.prof.pool.pop_back();
+      } else {
+        hipError_t e = hipEventCreate(&a);
+        if (e != hipSuccess) return e;
+        e = hipEventCreate(&b);
+        if (e != hipSuccess) return e;
+      }
+      (void)hipEventRecord(a, c.stream);
+      f();
+      (void)hipEventRecord(b, c.stream);
+      c.prof.pending.push_back({name, a, b});
+    } else {
+      f();
+    }
+    return hipGetLastError();
+  }
+};
+
+inline int blocks_for(long long n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t enqueue_reset_flags(Ctx& c) {
+  k_reset_flags<<<1, 64, 0, c.stream>>>(c.flags, c.steps_started);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_init_grid(Ctx& c) {
+  k_init_bbox<<<blocks_for(c.n_scenes, 256), 256, 0, c.stream>>>(c.n_scenes, c.bbox);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_upload(Ctx& c, const float* d_pos, const float* d_vel) {
+  if (c.n == 0) return hipSuccess;
+  k_upload<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(c.n, d_pos, d_vel, c.posA, c.velA);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel) {
+  if (c.n == 0) return hipSuccess;
+  k_download<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(c.n, c.posA, c.velA, d_pos, d_vel);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_substep(Ctx& c) {
+  if (c.n == 0) return hipSuccess;
+  Launcher L{c};
+  const StepConsts k = c.consts;
+  const int bs = c.block;
+  const int nb = blocks_for(c.n, bs);
+  hipStream_t st = c.stream;
+  hipError_t e;
+
+#define FH_TRY(x)                      \
+  do {                                 \
+    e = (x);                           \
+    if (e != hipSuccess) return e;     \
+  } while (0)
+
+  FH_TRY(L("predict_bbox", [&] {
+    k_predict_bbox<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.planes, c.seg, c.cellc, c.bbox,
+                                      c.flags);
+  }));
+  FH_TRY(L("grid_setup", [&] {
+    k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.bbox, c.grid, c.total_cells, c.cell_cap,
+                                    c.flags, c.nbr_total, c.need_cells, c.steps_started);
+  }));
+  FH_TRY(L("cell_count", [&] {
+    k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.rank,
+                                    c.flags);
+  }));
+  FH_TRY(L("scan", [&] {
+    k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
+                                                 c.flags);
+    k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
+    k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
+                                               c.start, 1, c.flags);
+  }));
+  FH_TRY(L("scatter", [&] {
+    k_scatter<<<nb, bs, 0, st>>>(k, c.key, c.rank, c.start, c.posA, c.perm, c.sid, c.skey,
+                                 c.flags);
+  }));
+  FH_TRY(L("reorder", [&] {
+    k_reorder<<<nb, bs, 0, st>>>(k, c.perm, c.sid, c.skey, c.start, c.seg, c.planes, c.posA,
+                                 c.velA, c.posB, c.velB, c.pred[0], c.cellc, c.diag_cell,
+                                 c.diag_key, c.flags);
+  }));
+  FH_TRY(L("neighbors", [&] {
+    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], c.seg, c.grid, c.start, c.nrange, c.nbr,
+                                   c.nbr_cap, c.nbr_total, c.flags);
+  }));
+
+  // velocity buffer that the commit writes: XSPH needs a second buffer to
+  // stay Jacobi, so the commit goes to velB and XSPH back to velA.
+  float4* v1 = c.xsph ? c.velB : c.velA;
+  const bool rest_in_delta = c.restitution_any && !c.xsph && !c.vort;
+  const int I = c.iterations;
+  for (int it = 0; it < I; ++it) {
+    float4* P = c.pred[it & 1];
+    float4* Pn = c.pred[(it + 1) & 1];
+    const bool last = (it + 1 == I);
+    FH_TRY(L("lambda", [&] {
+      k_lambda<<<nb, bs, 0, st>>>(k, P, c.nrange, c.nbr, c.rho, last ? 1 : 0, c.flags);
+    }));
+    FH_TRY(L("delta", [&] {
+      if (!last)
+        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+                                                       c.planes, c.posB, c.posA, v1, c.rho,
+                                                       c.flags);
+      else if (rest_in_delta)
+        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+                                                     c.planes, c.posB, c.posA, v1, c.rho,
+                                                     c.flags);
+      else
+        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+                                                      c.planes, c.posB, c.posA, v1, c.rho,
+                                                      c.flags);
+    }));
+  }
+  if (I == 0) {
+    FH_TRY(L("delta", [&] {
+      if (rest_in_delta)
+        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], c.nrange, c.nbr,
+                                                      c.seg, c.planes, c.posB, c.posA, v1,
+                                                      c.rho, c.flags);
+      else
+        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], c.nrange,
+                                                       c.nbr, c.seg, c.planes, c.posB, c.posA,
+                                                       v1, c.rho, c.flags);
+    }));
+  }
+  if (c.xsph) {
+    const bool rest = c.restitution_any && !c.vort;
+    FH_TRY(L("xsph", [&] {
+      if (rest)
+        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, c.nrange, c.nbr, c.seg,
+                                        c.planes, c.flags);
+      else
+        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, c.nrange, c.nbr, c.seg,
+                                         c.planes, c.flags);
+    }));
+  }
+  if (c.vort) {
+    FH_TRY(L("vorticity_omega", [&] {
+      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, c.nrange, c.nbr, c.flags);
+    }));
+    FH_TRY(L("vorticity_apply", [&] {
+      if (c.restitution_any)
+        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, c.nrange, c.nbr,
+                                              c.seg, c.planes, c.flags);
+      else
+        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, c.nrange, c.nbr,
+                                               c.seg, c.planes, c.flags);
+    }));
+  }
+#undef FH_TRY
+  return hipSuccess;
+}
+
+}  // namespace fluidhip

@@ -1,0 +1,182 @@
+/*
+ * fluid_hip.h — C ABI of the MI355X (gfx950) PBF backend.
+ *
+ * This is the drop-in boundary that replaces the reference's CUDA backend
+ * (cuda/include/fluid/cuda.h:7-9) behind the same solver contract
+ * (`fluid::Params`, `fluid::State`, core/include/fluid/core.h:11-132).  Plain
+ * C types only: no C++ exceptions cross it, every call returns an int status
+ * (0 = ok, negative = error) and fluid_hip_last_error() gives the message of
+ * the last failing call on the calling thread.
+ *
+ * Memory: the library owns all device memory of a context; the caller owns
+ * every host buffer it passes.  Positions and velocities cross the boundary as
+ * SoA blocks of 3*n floats (all x, then all y, then all z) in ORIGINAL
+ * particle order; on the device they stay resident, in the solver's own
+ * order, across steps.
+ *
+ * Threading: one host thread per context at a time.  Work is queued on the
+ * context's stream (default: a stream the context creates); calls that
+ * return data synchronize that stream.
+ */
+#ifndef FLUID_HIP_H
+#define FLUID_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLUID_HIP_ABI_VERSION 1
+#define FLUID_HIP_MAX_PLANES 8
+
+/* Status codes. */
+#define FLUID_HIP_OK 0
+#define FLUID_HIP_EINVAL -1       /* bad argument or parameter */
+#define FLUID_HIP_ENODEV -2       /* no usable HIP device */
+#define FLUID_HIP_EHIP -3         /* a HIP runtime call failed */
+#define FLUID_HIP_ECAPACITY -4    /* cell table or neighbor list overflowed */
+#define FLUID_HIP_ERANGE -5       /* non-finite or out-of-range position */
+#define FLUID_HIP_EUNSUPPORTED -6 /* valid in the reference, not offered here */
+#define FLUID_HIP_ESTATE -7       /* call out of order (e.g. no data uploaded) */
+
+/*
+ * Per-scene solver parameters: a flattened `fluid::Params`
+ * (core/include/fluid/core.h:11-79) with the plane list inline.  Fields the
+ * reference cannot set from JSON/CLI (solver_iterations, scorr_dq_coeff,
+ * vort_epsilon, vort_norm_eps) are ordinary fields here.  In a batch, every
+ * scene must agree on all fields except gravity, the planes, restitution and
+ * friction, which may differ per scene.
+ */
+typedef struct fluid_hip_params {
+  float dt;
+  float density;
+  float particle_mass;
+  float h;
+  float particle_radius; /* carried for completeness; the step never reads it */
+  float epsilon;
+  int32_t solver_iterations;
+  int32_t use_uniform_grid; /* must be 1: the O(N^2) path is CPU-only */
+  int32_t enable_scorr;
+  int32_t enable_xsph;
+  int32_t enable_vorticity;
+  float scorr_k;
+  int32_t scorr_n;
+  float scorr_dq_coeff;
+  float visc_c;
+  float plane_restitution;
+  float plane_friction;
+  float vort_epsilon;
+  float vort_norm_eps;
+  float gravity[3];          /* Params::external_forces */
+  int32_t n_planes;          /* <= FLUID_HIP_MAX_PLANES */
+  float planes[FLUID_HIP_MAX_PLANES][4]; /* (nx, ny, nz, d): n.p - d >= 0 inside */
+} fluid_hip_params;
+
+/* Capacity knobs for fluid_hip_create; zero fields take defaults. */
+typedef struct fluid_hip_config {
+  int64_t cell_capacity;      /* cells of the dense grid table, all scenes */
+  int64_t neighbor_capacity;  /* neighbor-list entries, all scenes */
+  int32_t device;             /* HIP device ordinal */
+  int32_t block_size;         /* threads per workgroup of the particle kernels (64..1024) */
+  int32_t xcd_remap;          /* 1: map consecutive particle tiles to one XCD */
+  int32_t record_diag;        /* 1: keep per-step grid diagnostics (slower) */
+} fluid_hip_config;
+
+typedef struct fluid_hip_ctx fluid_hip_ctx;
+
+/* ---- query (cuda/include/fluid/cuda.h:7-8) ---------------------------- */
+
+/* Replaces `int cuda_version()` (cuda/include/fluid/cuda.h:7). */
+int fluid_hip_version(void);
+
+/* Replaces `bool cuda_device_available(int*, const char**)`
+ * (cuda/include/fluid/cuda.h:8, cuda/src/cuda_stub.cu:740-762): returns 1
+ * when at least one device is usable, else 0 with *error (static string)
+ * set when the runtime reported an error. */
+int fluid_hip_device_available(int* count, const char** error);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* fluid_hip_last_error(void);
+
+/* ---- context ------------------------------------------------------------ */
+
+/*
+ * Creates a context for `n_scenes` independent scenes with
+ * scene_counts[s] particles each (scenes are concatenated in that order in
+ * every array).  One scene is the plain backend; more than one is the
+ * segmented batch, stepped by the same kernels in one launch each.
+ * `params` holds n_scenes entries.  No counterpart in the reference, whose
+ * CUDA step allocates and frees everything per call (cuda_stub.cu:791-816).
+ */
+int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene_counts,
+                     const fluid_hip_params* params, const fluid_hip_config* config);
+int fluid_hip_destroy(fluid_hip_ctx* ctx);
+
+/* Replace the per-scene parameters (same validation as create). */
+int fluid_hip_set_params(fluid_hip_ctx* ctx, const fluid_hip_params* params);
+
+/* Queue work on this stream (a hipStream_t) instead of the context's own. */
+int fluid_hip_set_stream(fluid_hip_ctx* ctx, void* stream);
+
+/* ---- state transfer (the H2D/D2H of cuda_stub.cu:791-816, :1092-1097) --- */
+
+/* pos, vel: host SoA blocks of 3*n floats, original particle order. */
+int fluid_hip_upload(fluid_hip_ctx* ctx, const float* pos, const float* vel, float time);
+int fluid_hip_download(fluid_hip_ctx* ctx, float* pos, float* vel, float* time);
+
+/* ---- stepping ------------------------------------------------------------ */
+
+/*
+ * Advances every scene by `nsteps` substeps of `fluid::step`
+ * (core/src/core.cpp:119-615; the reference's GPU form is `cuda_step`,
+ * cuda/src/cuda_stub.cu:764-1099), device-resident.  Synchronizes once at
+ * the end to check the device error flags.  use_graph=1 replays a captured
+ * hipGraph of one substep.
+ */
+int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph);
+
+/* Queue `nsteps` substeps without synchronizing or checking errors (for
+ * timing loops); fluid_hip_sync() checks afterwards. */
+int fluid_hip_step_async(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph);
+int fluid_hip_sync(fluid_hip_ctx* ctx);
+
+/* ---- diagnostics of the last step (original particle order) -------------- */
+
+/*
+ * cell[3*n]: integer cell (x,y,z) of the predicted position pred0, as
+ *   floor(p * (1/h)) (core/src/core.cpp:28-34).  Needs record_diag=1.
+ * sorted_ids[n]: original ids in the solver's sorted order (scene, Z-order
+ *   cell key, original id).  Needs record_diag=1 for the cell keys, not ids.
+ * nbr_count[n]: neighbor count of each particle (core/src/core.cpp:205-247).
+ * rho[n], lambda[n]: density and lambda of the last constraint iteration.
+ * Any pointer may be NULL.
+ */
+int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
+                       int32_t* nbr_count, float* rho, float* lambda);
+
+/* Neighbor lists of the last step as original ids, particle by particle in
+ * original order, each list in the reference's order (dz, dy, dx cells,
+ * ascending id).  `cap` entries of `ids`; returns the total in *total. */
+int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* ids, int64_t cap, int64_t* total);
+
+/* Sorted cell key (scene offset + brick-Z-order index) of each sorted slot and
+ * the scene grid descriptors, for the bit-exact sort tests.  desc receives 8
+ * int32 per scene: origin x,y,z, bricks x,y,z, cell offset, cell count. */
+int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
+
+/* ---- timing -------------------------------------------------------------- */
+
+/* Per-kernel device time (HIP events on the context's stream) of the steps
+ * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_count",
+ * "scan", "scatter", "reorder", "neighbors", "lambda", "delta", "xsph",
+ * "vorticity_omega", "vorticity_apply", "finalize". */
+int fluid_hip_profile(fluid_hip_ctx* ctx, int32_t enable);
+int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total_ms,
+                           int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUID_HIP_H */

@@ -1,0 +1,190 @@
+"""HIP backend vs the CPU oracle and the reference's golden vectors, through
+the C ABI.  Bar: cell keys, sorted order, neighbor counts and lists
+bit-exact; positions and velocities within 1e-5*h after one step (the
+kernels follow the reference's operation order, so they are compared
+bitwise as well)."""
+import numpy as np
+import pytest
+
+import oracle
+from fluidsimulator_amd import hip, scenes
+from tests.helpers import golden_names, load_golden, neighbor_lists, same_bits
+
+pytestmark = pytest.mark.gpu
+
+H = 0.1
+POS_TOL = 1e-5 * H
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    hip.load_library()  # raises when the library is missing: no fallback
+    ok, count, err = hip.hip_device_available()
+    assert ok, f"no HIP device: {err}"
+    oracle.build(ref=False)
+
+
+def run_hip(p, pos, vel, steps, record_diag=True, graph=False, **kw):
+    with hip.HipSolver(p, [pos.shape[1]], record_diag=record_diag, **kw) as s:
+        s.upload(pos, vel, 0.0)
+        s.step(steps, graph=graph)
+        st = s.download()
+        out = {"pos": st.pos, "vel": st.vel, "time": np.float32(st.time)}
+        if steps > 0:
+            out.update(s.diag())
+            out["nbr_ids"] = s.neighbors()
+            out["keys"], out["desc"] = s.grid()
+    return out
+
+
+def brick_keys(cells, desc):
+    """Sort key of each cell: scene offset + brick index * 64 + Z-order of
+    the cell inside its 4x4x4 brick (same definition as the kernels)."""
+    ox, oy, oz, nbx, nby, nbz, off, _ = [int(v) for v in desc]
+    rx, ry, rz = cells[:, 0] - ox, cells[:, 1] - oy, cells[:, 2] - oz
+    b = ((rz >> 2) * nby + (ry >> 2)) * nbx + (rx >> 2)
+    m = (rx & 1) | ((ry & 1) << 1) | ((rz & 1) << 2) | ((rx & 2) << 2) | ((ry & 2) << 3) | \
+        ((rz & 2) << 4)
+    return off + b * 64 + m
+
+
+def expected_desc(cells):
+    lo = cells.min(axis=0)
+    hi = cells.max(axis=0)
+    size = hi - lo + 3
+    nb = (size + 3) // 4
+    return np.array([lo[0] - 1, lo[1] - 1, lo[2] - 1, nb[0], nb[1], nb[2], 0,
+                     int(nb.prod()) * 64], np.int64)
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names() if not n.startswith("bruteforce")])
+def test_golden_bitwise(name):
+    p, exp = load_golden(name)
+    r = run_hip(p, exp["pos0"], exp["vel0"], int(exp["steps"]))
+    assert same_bits(r["pos"], exp["pos"])
+    assert same_bits(r["vel"], exp["vel"])
+    assert same_bits(r["time"], exp["time"])
+    assert np.array_equal(r["nbr_count"], exp["nbr_count"])
+    assert np.array_equal(r["nbr_ids"], exp["nbr_ids"])
+    assert np.array_equal(r["cell"], exp["cell"])
+    assert same_bits(r["rho"], exp["rho"])
+    assert same_bits(r["lambda"], exp["lambda"])
+
+
+def test_bruteforce_path_is_refused():
+    p, exp = load_golden("bruteforce_120_1step")
+    with pytest.raises(hip.HipError) as ei:
+        hip.HipSolver(p, [exp["pos0"].shape[1]])
+    assert ei.value.code == hip.EUNSUPPORTED
+
+
+@pytest.mark.parametrize("flags", [False, True])
+def test_dam_break_32k_grid_bit_exact(flags):
+    sc = scenes.dam_break(seed=7, flags=flags)
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=1, diag=True)
+    r = run_hip(sc.params, sc.state.pos, sc.state.vel, 1)
+    # cell keys (integer cells of pred0)
+    assert np.array_equal(r["cell"], o["cell"])
+    # grid descriptor and sorted order: (scene, brick Z-order key, id)
+    desc = expected_desc(o["cell"])
+    assert np.array_equal(r["desc"][0], desc)
+    keys = brick_keys(o["cell"], desc)
+    order = np.lexsort((np.arange(sc.state.n), keys))
+    assert np.array_equal(r["sorted_ids"], order)
+    assert np.array_equal(r["keys"], keys[order])
+    # neighbor counts and lists, in the reference's order
+    assert np.array_equal(r["nbr_count"], o["nbr_count"])
+    assert np.array_equal(r["nbr_ids"], o["nbr_ids"])
+    # one step: within 1e-5 h, and in fact bitwise
+    assert np.abs(r["pos"] - o["pos"]).max() <= POS_TOL
+    assert same_bits(r["pos"], o["pos"])
+    assert same_bits(r["vel"], o["vel"])
+    assert same_bits(r["rho"], o["rho"])
+    assert same_bits(r["lambda"], o["lambda"])
+
+
+def test_double_dam_break_all_flags_262k_one_step():
+    sc = scenes.double_dam_break()
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=1, diag=True)
+    r = run_hip(sc.params, sc.state.pos, sc.state.vel, 1)
+    assert np.array_equal(r["cell"], o["cell"])
+    assert np.array_equal(r["nbr_count"], o["nbr_count"])
+    assert np.abs(r["pos"] - o["pos"]).max() <= POS_TOL
+    assert np.abs(r["vel"] - o["vel"]).max() <= POS_TOL / float(sc.params.dt)
+    assert same_bits(r["pos"], o["pos"])
+    assert same_bits(r["vel"], o["vel"])
+
+
+def test_multi_step_rollout_bitwise_and_density_stats():
+    # 60 steps of the all-flags dam break: the kernels repeat the
+    # reference's arithmetic exactly, so the rollout stays bitwise equal;
+    # the density-error statistic is compared as well.
+    sc = scenes.dam_break(seed=8, dims=(24, 24, 24), flags=True)
+    steps = 60
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=steps, diag=True)
+    r = run_hip(sc.params, sc.state.pos, sc.state.vel, steps)
+    err_o = np.mean(np.abs(o["rho"] / 1000.0 - 1.0))
+    err_r = np.mean(np.abs(r["rho"] / 1000.0 - 1.0))
+    assert abs(err_r - err_o) <= 0.01 * err_o
+    assert same_bits(r["pos"], o["pos"])
+    assert same_bits(r["vel"], o["vel"])
+
+
+def test_graph_replay_equals_eager():
+    sc = scenes.dam_break(seed=9, dims=(20, 20, 20), flags=True)
+    a = run_hip(sc.params, sc.state.pos, sc.state.vel, 5, graph=False)
+    b = run_hip(sc.params, sc.state.pos, sc.state.vel, 5, graph=True)
+    assert same_bits(a["pos"], b["pos"]) and same_bits(a["vel"], b["vel"])
+
+
+@pytest.mark.parametrize("block,remap", [(64, 0), (128, 1), (512, 1), (1024, 0)])
+def test_launch_shapes_do_not_change_results(block, remap):
+    sc = scenes.dam_break(seed=10, dims=(20, 16, 12), flags=True)
+    a = run_hip(sc.params, sc.state.pos, sc.state.vel, 3)
+    b = run_hip(sc.params, sc.state.pos, sc.state.vel, 3, block_size=block, xcd_remap=remap)
+    assert same_bits(a["pos"], b["pos"]) and same_bits(a["vel"], b["vel"])
+
+
+def test_capacity_auto_growth_keeps_results():
+    sc = scenes.dam_break(seed=11, dims=(16, 16, 16), flags=True)
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=4)
+    r = run_hip(sc.params, sc.state.pos, sc.state.vel, 4, cell_capacity=64,
+                neighbor_capacity=64)
+    assert same_bits(r["pos"], o["pos"]) and same_bits(r["vel"], o["vel"])
+
+
+def test_non_finite_position_reports_range_error_and_keeps_state():
+    sc = scenes.dam_break(seed=12, dims=(8, 8, 8))
+    pos = sc.state.pos.copy()
+    pos[1, 17] = np.nan
+    with hip.HipSolver(sc.params, [sc.state.n]) as s:
+        s.upload(pos, sc.state.vel, 0.0)
+        with pytest.raises(hip.HipError) as ei:
+            s.step(1)
+        assert ei.value.code == hip.ERANGE
+        st = s.download()
+    assert same_bits(st.vel, sc.state.vel)
+    assert np.isnan(st.pos[1, 17])
+
+
+def test_empty_and_single_particle():
+    p = scenes.base_params((1, 1, 1), True)
+    with hip.HipSolver(p, [0]) as s:
+        s.upload(np.zeros((3, 0), np.float32), np.zeros((3, 0), np.float32), 0.0)
+        s.step(3)
+        assert s.download().time == pytest.approx(3 * float(np.float32(p.dt)), rel=1e-6)
+    x = np.array([[0.5], [0.5], [0.5]], np.float32)
+    v = np.array([[0.1], [0.2], [0.3]], np.float32)
+    o = oracle.step(p, x, v, nsteps=2)
+    r = run_hip(p, x, v, 2, record_diag=False)
+    assert same_bits(r["pos"], o["pos"]) and same_bits(r["vel"], o["vel"])
+
+
+def test_hip_step_stateless_call_shape():
+    # hip_step(params, state) has the call shape of cuda_step(params, state)
+    sc = scenes.dam_break(seed=13, dims=(10, 10, 10), flags=True)
+    st = sc.state.copy()
+    hip.hip_step(sc.params, st)
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=1)
+    assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"])
+    assert st.time == pytest.approx(float(o["time"]))
