@@ -105,8 +105,7 @@ def main():
         s.profile(True)
     sync_all()
     t0 = time.perf_counter()
-    s.step_async(args.steps, graph=args.graph)
-    s.sync()
+    s.step(args.steps, graph=args.graph)  # one sync at the end; grows tables if needed
     sync_all()
     elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed

@@ -125,6 +125,11 @@ void compute_consts(Ctx& c) {
   k.w0 = host_poly6(0.0f, h);
   const float h6 = h2 * h2 * h2;
   k.s_coeff = -45.0f / (kPi * h6);
+  {
+    // spiky_gradient_factor(sqrt(min_r2), h) as core.cpp:48-57 evaluates it
+    const float r = std::sqrt(k.min_r2);
+    k.f_min = (r > h) ? 0.0f : k.s_coeff * (h - r) * (h - r);
+  }
   k.inv_density = 1.0f / p.density;
   k.mass = p.particle_mass;
   k.grad_scale = p.particle_mass * k.inv_density;
@@ -199,12 +204,15 @@ void free_all(Ctx& c) {
   dfree(c.rho);
   dfree(c.cellc);
   for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
-                  &c.total_cells, &c.scan_partial, &c.diag_key, &c.nbr, &c.seg,
+                  &c.total_cells, &c.scan_partial, &c.diag_key, &c.nbr, &c.seg, &c.nwords,
+                  &c.ncount, &c.tile_bbox,
                   &c.steps_started})
     dfree(*p);
   dfree(c.diag_cell);
   dfree(c.grid);
-  dfree(c.nrange);
+  dfree(c.noff);
+  dfree(c.nwords);
+  dfree(c.ncount);
   dfree(c.nbr_total);
   dfree(c.need_cells);
   dfree(c.planes);
@@ -408,8 +416,11 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
     ALLOC(c.diag_cell, n);
     ALLOC(c.diag_key, n);
   }
-  ALLOC(c.nrange, n);
+  ALLOC(c.noff, n);
+  ALLOC(c.nwords, n);
+  ALLOC(c.ncount, n);
   ALLOC(c.bbox, 6 * static_cast<size_t>(n_scenes));
+  ALLOC(c.tile_bbox, 6 * (n / 64 + 1));
   ALLOC(c.grid, static_cast<size_t>(n_scenes));
   ALLOC(c.total_cells, 1);
   ALLOC(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
@@ -425,7 +436,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
       cfg.cell_capacity > 0 ? cfg.cell_capacity : std::max<long long>(1ll << 22, 8ll * c.n);
   const long long nbr_cap =
       cfg.neighbor_capacity > 0 ? cfg.neighbor_capacity : std::max<long long>(1ll << 16, 40ll * c.n);
-  if (cell_cap >= INT_MAX || nbr_cap >= INT_MAX) return bail(fail(FLUID_HIP_EINVAL, "capacity >= 2^31"));
+  if (cell_cap >= INT_MAX) return bail(fail(FLUID_HIP_EINVAL, "cell capacity >= 2^31"));
   if ((rc = alloc_cells(c, cell_cap))) return bail(rc);
   if ((rc = alloc_neighbors(c, nbr_cap))) return bail(rc);
   e = hipMemcpyAsync(c.seg, c.seg_h.data(), sizeof(int) * (n_scenes + 1), hipMemcpyHostToDevice,
@@ -586,8 +597,10 @@ int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
     } else {
       const long long need = static_cast<long long>(st.nbr_total);
       const long long cap = need + need / 4 + 4096;
-      if (cap >= INT_MAX) return fail(FLUID_HIP_ECAPACITY, "neighbor list would exceed 2^31 entries");
-      if ((rc = alloc_neighbors(c, cap))) return rc;
+      if ((rc = alloc_neighbors(c, cap))) {
+        g_last_error = "neighbor list of " + std::to_string(cap) + " words: " + g_last_error;
+        return FLUID_HIP_ECAPACITY;
+      }
     }
     drop_graph(c);
   }
@@ -625,9 +638,9 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
     }
   }
   if (nbr_count) {
-    std::vector<int2> nr(n);
-    HIP_TRY(hipMemcpy(nr.data(), c.nrange, sizeof(int2) * n, hipMemcpyDeviceToHost), "read ranges");
-    for (size_t s = 0; s < n; ++s) nbr_count[ids[s]] = nr[s].y;
+    std::vector<int> nc(n);
+    HIP_TRY(hipMemcpy(nc.data(), c.ncount, sizeof(int) * n, hipMemcpyDeviceToHost), "read counts");
+    for (size_t s = 0; s < n; ++s) nbr_count[ids[s]] = nc[s];
   }
   if (rho) {
     std::vector<float> r(n);
@@ -655,9 +668,11 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
   if (!c.stepped) return fail(FLUID_HIP_ESTATE, "no step taken since upload");
   const size_t n = static_cast<size_t>(c.n);
   std::vector<float4> pos(n);
-  std::vector<int2> nr(n);
+  std::vector<long long> off(n);
+  std::vector<int> nw(n);
   HIP_TRY(hipMemcpy(pos.data(), c.posA, sizeof(float4) * n, hipMemcpyDeviceToHost), "read pos");
-  HIP_TRY(hipMemcpy(nr.data(), c.nrange, sizeof(int2) * n, hipMemcpyDeviceToHost), "read ranges");
+  HIP_TRY(hipMemcpy(off.data(), c.noff, sizeof(long long) * n, hipMemcpyDeviceToHost), "read offsets");
+  HIP_TRY(hipMemcpy(nw.data(), c.nwords, sizeof(int) * n, hipMemcpyDeviceToHost), "read lengths");
   unsigned long long used = 0;
   HIP_TRY(hipMemcpy(&used, c.nbr_total, sizeof(used), hipMemcpyDeviceToHost), "read total");
   used = std::min<unsigned long long>(used, static_cast<unsigned long long>(c.nbr_cap));
@@ -673,10 +688,17 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
   }
   int64_t k = 0;
   for (size_t id = 0; id < n; ++id) {
-    const int2 r = nr[slot_of[id]];
-    for (int j = 0; j < r.y; ++j) {
-      if (out && k < cap) out[k] = ids[list[r.x + j]];
-      ++k;
+    const int s = slot_of[id];
+    const int* L = list.data() + off[s];
+    for (int w = 0; w < nw[s];) {
+      const int e = L[w++];
+      int t = e & 0x7fffffff;
+      int end = t + 1;
+      if (e < 0) end = t + L[w++];
+      for (; t < end; ++t) {
+        if (out && k < cap) out[k] = ids[t];
+        ++k;
+      }
     }
   }
   if (total) *total = k;
@@ -697,6 +719,27 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc) {
     HIP_TRY(hipMemcpy(desc, c.grid, sizeof(fluidhip::GridDesc) * c.n_scenes, hipMemcpyDeviceToHost),
             "read grid");
   }
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!out || n < 0) return fail(FLUID_HIP_EINVAL, "bad output");
+  int64_t v[5] = {0, 0, 0, c.nbr_cap, c.cell_cap};
+  if (c.stepped && c.n > 0) {
+    std::vector<int> cnt(c.n);
+    HIP_TRY(hipMemcpy(cnt.data(), c.ncount, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read counts");
+    for (int x : cnt) v[0] += x;
+    unsigned long long words = 0;
+    HIP_TRY(hipMemcpy(&words, c.nbr_total, sizeof(words), hipMemcpyDeviceToHost), "read words");
+    v[1] = static_cast<int64_t>(words);
+    int cells = 0;
+    HIP_TRY(hipMemcpy(&cells, c.total_cells, sizeof(int), hipMemcpyDeviceToHost), "read cells");
+    v[2] = cells;
+  }
+  for (int i = 0; i < n && i < 5; ++i) out[i] = v[i];
   return FLUID_HIP_OK;
 }
 

@@ -64,6 +64,7 @@ struct Ctx {
   int* start = nullptr;  // cell_cap + 1
   long long cell_cap = 0;
   int* bbox = nullptr;   // n_scenes * 6
+  int* tile_bbox = nullptr;  // 6 per particle tile (workgroup)
   GridDesc* grid = nullptr;
   int* total_cells = nullptr;
   int* scan_partial = nullptr;  // scan_blocks + 1
@@ -71,8 +72,10 @@ struct Ctx {
   int* diag_key = nullptr;
 
   // neighbors
-  int2* nrange = nullptr;
-  int* nbr = nullptr;
+  long long* noff = nullptr;  // first word of each slot's list
+  int* nwords = nullptr;      // encoded length (words)
+  int* ncount = nullptr;      // neighbors (decoded length)
+  int* nbr = nullptr;         // run-length-encoded lists, nbr_cap words
   long long nbr_cap = 0;
   unsigned long long* nbr_total = nullptr;
   unsigned long long* need_cells = nullptr;  // cells a failed substep needed

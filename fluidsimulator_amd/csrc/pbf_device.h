@@ -32,6 +32,7 @@ struct StepConsts {
   float inv_h;      // 1.0f / h            (core.cpp:29)
   float h2;         // h * h               (core.cpp:140)
   float min_r2;     // (0.01f*h)^2         (core.cpp:141-142)
+  float f_min;      // spiky(sqrt(min_r2)): the factor of every clamped pair
   float w_coeff;    // 315 / (64 pi h^9)   (core.cpp:44)
   float w0;         // poly6(0, h)         (core.cpp:319)
   float s_coeff;    // -45 / (pi h^6)      (core.cpp:55)

@@ -177,9 +177,11 @@ __global__ void k_init_bbox(int n_scenes, int* bbox) {
 __global__ __launch_bounds__(1024) void k_predict_bbox(
     StepConsts c, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const ScenePlanes* __restrict__ sp, const int* __restrict__ seg,
-    uint64_t* __restrict__ cellc, int* __restrict__ bbox, unsigned* __restrict__ flags) {
+    uint64_t* __restrict__ cellc, int* __restrict__ bbox, int* __restrict__ tile_bbox,
+    unsigned* __restrict__ flags) {
   __shared__ int red[6][1024 / kWave];
-  const int base = tile_index(c.xcd_remap) * blockDim.x;
+  const int tile = tile_index(c.xcd_remap);
+  const int base = tile * blockDim.x;
   const int i = base + threadIdx.x;
   const bool active = i < c.n;
   int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX;
@@ -226,20 +228,30 @@ __global__ __launch_bounds__(1024) void k_predict_bbox(
         mn0 = min(mn0, red[0][k]), mn1 = min(mn1, red[1][k]), mn2 = min(mn2, red[2][k]);
         mx0 = max(mx0, red[3][k]), mx1 = max(mx1, red[4][k]), mx2 = max(mx2, red[5][k]);
       }
-      int* b = bbox + 6 * s_first;
+      // one plain store per tile; grid_setup reduces the tiles of a scene
+      int* b = tile_bbox + 6 * tile;
+      b[0] = mn0, b[1] = mn1, b[2] = mn2, b[3] = mx0, b[4] = mx1, b[5] = mx2;
+    }
+  } else {
+    // a tile that straddles scenes: per-particle atomics (rare)
+    if (active) {
+      int* b = bbox + 6 * sc;
       atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
       atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
     }
-  } else if (active) {
-    int* b = bbox + 6 * sc;
-    atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
-    atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+    if (threadIdx.x == 0) {
+      int* b = tile_bbox + 6 * tile;
+      b[0] = b[1] = b[2] = INT_MAX;
+      b[3] = b[4] = b[5] = INT_MIN;
+    }
   }
 }
 
 // One workgroup: per-scene table extents and key offsets; resets the
 // per-step counters.
-__global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, int* __restrict__ bbox,
+__global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __restrict__ seg,
+                                                    const int* __restrict__ tile_bbox,
+                                                    int tile_size, int* __restrict__ bbox,
                                                     GridDesc* __restrict__ grid,
                                                     int* __restrict__ total_cells,
                                                     long long cell_cap,
@@ -253,7 +265,37 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, int* __restric
   // the retry; only the bounding boxes are reset.
   const bool was_fatal = (*flags & kFlagFatal) != 0u;
   if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
+  __shared__ int red[6][256 / kWave];
+  // Fold each scene's tile partials into its box (which already holds the
+  // atomics of tiles that straddle two scenes).
+  for (int sc = 0; sc < n_scenes; ++sc) {
+    const int a = seg[sc], b = seg[sc + 1];
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    if (b > a) {
+      for (int t = a / tile_size + static_cast<int>(threadIdx.x); t <= (b - 1) / tile_size;
+           t += blockDim.x) {
+        const int* p = tile_bbox + 6 * t;
+        v[0] = min(v[0], p[0]), v[1] = min(v[1], p[1]), v[2] = min(v[2], p[2]);
+        v[3] = max(v[3], p[3]), v[4] = max(v[4], p[4]), v[5] = max(v[5], p[5]);
+      }
+    }
+    for (int k = 0; k < 3; ++k) v[k] = wave_min(v[k]);
+    for (int k = 3; k < 6; ++k) v[k] = wave_max(v[k]);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) red[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < static_cast<int>(blockDim.x) / kWave; ++q) {
+        for (int k = 0; k < 3; ++k) v[k] = min(v[k], red[k][q]);
+        for (int k = 3; k < 6; ++k) v[k] = max(v[k], red[k][q]);
+      }
+      int* bb = bbox + 6 * sc;
+      for (int k = 0; k < 3; ++k) bb[k] = min(bb[k], v[k]);
+      for (int k = 3; k < 6; ++k) bb[k] = max(bb[k], v[k]);
+    }
+    __syncthreads();
+  }
   for (int b0 = 0; b0 < n_scenes; b0 += blockDim.x) {
     const int s = b0 + threadIdx.x;
     long long nc = 0;
@@ -438,12 +480,50 @@ __global__ __launch_bounds__(1024) void k_reorder(
 
 // ---------------------------------------------------------------- neighbors
 
+// Neighbor lists are run-length encoded in slot space: a word t >= 0 is the
+// single neighbor t; a word with the top bit set starts a run of
+// consecutive slots (t & 0x7fffffff) .. + next word - 1.  Iterating the
+// words in order visits neighbors in exactly the order they were found
+// (27 cells in (dz, dy, dx) order, each by ascending id), so the encoding is
+// invisible to the arithmetic.  Dense cells — where particles pile up on
+// walls and corners — produce long runs and shrink by orders of magnitude.
+struct RunWriter {
+  int* out;
+  long long words = 0;
+  int count = 0;
+  int run_start = -1;
+  int run_len = 0;
+  template <bool kFill>
+  __device__ __forceinline__ void flush() {
+    if (run_len == 1) {
+      if (kFill) out[words] = run_start;
+      words += 1;
+    } else if (run_len > 1) {
+      if (kFill) {
+        out[words] = run_start | static_cast<int>(0x80000000u);
+        out[words + 1] = run_len;
+      }
+      words += 2;
+    }
+  }
+  template <bool kFill>
+  __device__ __forceinline__ void add(int t) {
+    ++count;
+    if (run_len > 0 && t == run_start + run_len) {
+      ++run_len;
+      return;
+    }
+    flush<kFill>();
+    run_start = t;
+    run_len = 1;
+  }
+};
+
 template <bool kFill>
-__device__ __forceinline__ int walk_cells(int s, float px, float py, float pz, int rx, int ry,
+__device__ __forceinline__ void walk_cells(int s, float px, float py, float pz, int rx, int ry,
                                           int rz, const GridDesc& g, const StepConsts& c,
                                           const int* __restrict__ start,
-                                          const float4* __restrict__ pred, int* __restrict__ out) {
-  int cnt = 0;
+                                          const float4* __restrict__ pred, RunWriter& w) {
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
@@ -457,25 +537,23 @@ __device__ __forceinline__ int walk_cells(int s, float px, float py, float pz, i
           const float ey = py - q.y;
           const float ez = pz - q.z;
           const float r2 = ex * ex + ey * ey + ez * ez;
-          if (r2 < c.h2) {
-            if (kFill) out[cnt] = t;
-            ++cnt;
-          }
+          if (r2 < c.h2) w.add<kFill>(t);
         }
       }
-  return cnt;
+  w.flush<kFill>();
 }
 
 __global__ __launch_bounds__(1024) void k_neighbors(
     StepConsts c, const float4* __restrict__ pred, const int* __restrict__ seg,
-    const GridDesc* __restrict__ grid, const int* __restrict__ start, int2* __restrict__ nrange,
+    const GridDesc* __restrict__ grid, const int* __restrict__ start,
+    long long* __restrict__ noff, int* __restrict__ nwords, int* __restrict__ ncount,
     int* __restrict__ nbr, long long nbr_cap, unsigned long long* __restrict__ nbr_total,
     unsigned* __restrict__ flags) {
   __shared__ int lds[1024 / kWave + 1];
   __shared__ long long base_sh;
   __shared__ int abort_sh;
   // Other workgroups of this launch may raise the neighbor flag at any time:
-  // read it once per workgroup so the barrier below stays uniform.
+  // read it once per workgroup so the barriers below stay uniform.
   if (threadIdx.x == 0) abort_sh = fatal(flags) ? 1 : 0;
   __syncthreads();
   if (abort_sh) return;
@@ -484,7 +562,7 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   float px = 0.f, py = 0.f, pz = 0.f;
   int rx = 0, ry = 0, rz = 0;
   GridDesc g{};
-  int cnt = 0;
+  RunWriter w{nullptr};
   if (active) {
     const float3 p = ld3(pred, s);
     px = p.x, py = p.y, pz = p.z;
@@ -500,58 +578,127 @@ __global__ __launch_bounds__(1024) void k_neighbors(
       atomicOr(flags, kFlagRange);
       active = false;
     } else {
-      cnt = walk_cells<false>(s, px, py, pz, rx, ry, rz, g, c, start, pred, nullptr);
+      walk_cells<false>(s, px, py, pz, rx, ry, rz, g, c, start, pred, w);
     }
   }
+  // The encoded length of one list stays below 2^31 words (at most two words
+  // per neighbor and fewer than 2^29 particles).
+  const int nw = static_cast<int>(w.words);
   int total;
-  const int pre = block_excl_scan(cnt, lds, &total);
+  const int pre = block_excl_scan(nw, lds, &total);
   if (threadIdx.x == 0)
     base_sh = static_cast<long long>(atomicAdd(nbr_total, static_cast<unsigned long long>(total)));
   __syncthreads();
-  if (!active) return;
-  const long long off = base_sh + pre;
-  if (off + cnt > nbr_cap) {
-    atomicOr(flags, kFlagNeighbors);
-    nrange[s] = make_int2(0, 0);
+  if (!active) {
+    if (s < c.n) {
+      noff[s] = 0;
+      nwords[s] = 0;
+      ncount[s] = 0;
+    }
     return;
   }
-  walk_cells<true>(s, px, py, pz, rx, ry, rz, g, c, start, pred, nbr + off);
-  nrange[s] = make_int2(static_cast<int>(off), cnt);
+  const long long off = base_sh + pre;
+  if (off + nw > nbr_cap) {
+    atomicOr(flags, kFlagNeighbors);
+    noff[s] = 0;
+    nwords[s] = 0;
+    ncount[s] = 0;
+    return;
+  }
+  RunWriter f{nbr + off};
+  walk_cells<true>(s, px, py, pz, rx, ry, rz, g, c, start, pred, f);
+  noff[s] = off;
+  nwords[s] = nw;
+  ncount[s] = f.count;
 }
 
+struct NbrView {
+  const long long* __restrict__ off;
+  const int* __restrict__ nw;
+  const int* __restrict__ list;
+};
+
+// Calls f(t, load(t)) for each neighbor slot t of slot s, in list order.
+// Runs are walked four slots at a time with the four loads issued before
+// any use, which keeps several memory requests in flight per lane; the
+// callbacks still run strictly in list order, so every sum accumulates in
+// the reference's order.
+template <typename Load, typename F>
+__device__ __forceinline__ void for_each_neighbor(const NbrView& v, int s, Load&& load, F&& f) {
+  const int* __restrict__ L = v.list + v.off[s];
+  const int nw = v.nw[s];
+  for (int w = 0; w < nw;) {
+    const int e = L[w++];
+    int t = e & 0x7fffffff;
+    int end = t + 1;
+    if (e < 0) end = t + L[w++];
+    for (; t + 4 <= end; t += 4) {
+      const auto a = load(t);
+      const auto b = load(t + 1);
+      const auto c = load(t + 2);
+      const auto d = load(t + 3);
+      f(t, a);
+      f(t + 1, b);
+      f(t + 2, c);
+      f(t + 3, d);
+    }
+    for (; t < end; ++t) f(t, load(t));
+  }
+}
+
+// Spiky factor of a pair at squared distance r2 < h2: pairs closer than
+// min_r share one precomputed factor (core.cpp:303-304 clamps r there), so
+// only the others pay for the IEEE square root.
+__device__ __forceinline__ float pair_spiky(float r2, const StepConsts& c) {
+  if (r2 < c.min_r2) return c.f_min;
+  return spiky(sqrtf(r2), c);
+}
+
+struct Pair2 {
+  float4 a, b;
+};
+
 // ---------------------------------------------------------------- solver
+//
+// Coincident pairs (dx = dy = dz = 0) are common where particles are pushed
+// onto walls and corners.  Their gradient terms are signed zeros; the
+// accumulators start at +0 and can never become -0 (x + y rounds an exact
+// cancellation to +0), so adding a signed zero leaves them bit-for-bit
+// unchanged and those additions are skipped.  Only finite products are
+// skipped: a non-finite factor times zero is NaN and is added as the
+// reference adds it.
 
 // lambda pass (core.cpp:281-329).  Writes lambda into pred[s].w.
 __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restrict__ P,
-                                                 const int2* __restrict__ nrange,
-                                                 const int* __restrict__ nbr,
-                                                 float* __restrict__ rho_out, int write_rho,
-                                                 const unsigned* __restrict__ flags) {
+                                                 NbrView nv, float* __restrict__ rho_out,
+                                                 int write_rho, const unsigned* __restrict__ flags) {
   const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (s >= c.n || fatal(flags)) return;
-  const int2 nr = nrange[s];
   const float3 p = ld3(P, s);
   float rho = 0.0f, gsx = 0.0f, gsy = 0.0f, gsz = 0.0f, sg2 = 0.0f;
   const float ngs = -c.grad_scale;
-  for (int j = 0; j < nr.y; ++j) {
-    const int t = nbr[nr.x + j];
-    const float3 q = ld3(P, t);
-    const float dx = p.x - q.x;
-    const float dy = p.y - q.y;
-    const float dz = p.z - q.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    rho += poly6(r2, c);
-    if (r2 < c.h2) {
-      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
-      const float f = spiky(r, c);
-      const float gx = f * dx, gy = f * dy, gz = f * dz;
-      gsx += gx;
-      gsy += gy;
-      gsz += gz;
-      const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
-      sg2 += jx * jx + jy * jy + jz * jz;
-    }
-  }
+  for_each_neighbor(
+      nv, s, [&](int t) { return ld3(P, t); },
+      [&](int, const float3& q) {
+        const float dx = p.x - q.x;
+        const float dy = p.y - q.y;
+        const float dz = p.z - q.z;
+        if (dx == 0.0f && dy == 0.0f && dz == 0.0f) {
+          rho += c.w0;  // poly6(0); f * (+-0) terms leave the sums unchanged
+          return;
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        rho += poly6(r2, c);
+        if (r2 < c.h2) {
+          const float f = pair_spiky(r2, c);
+          const float gx = f * dx, gy = f * dy, gz = f * dz;
+          gsx += gx;
+          gsy += gy;
+          gsz += gz;
+          const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
+          sg2 += jx * jx + jy * jy + jz * jz;
+        }
+      });
   rho += c.w0;
   rho *= c.mass;
   const float C = rho * c.inv_density - 1.0f;
@@ -566,41 +713,41 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
 // zero-iteration commit.  kRest fuses plane restitution (core.cpp:573-612).
 template <bool kSolve, bool kLast, bool kRest>
 __global__ __launch_bounds__(1024) void k_delta(
-    StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext,
-    const int2* __restrict__ nrange, const int* __restrict__ nbr, const int* __restrict__ seg,
-    const ScenePlanes* __restrict__ sp, const float4* __restrict__ posB,
-    float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
-    const unsigned* __restrict__ flags) {
+    StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext, NbrView nv,
+    const int* __restrict__ seg, const ScenePlanes* __restrict__ sp,
+    const float4* __restrict__ posB, float4* __restrict__ posA, float4* __restrict__ vout,
+    const float* __restrict__ rho, const unsigned* __restrict__ flags) {
   const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (s >= c.n || fatal(flags)) return;
   const float4 pi = P[s];
   float nx = pi.x, ny = pi.y, nz = pi.z;
   const ScenePlanes& sc = sp[scene_of(s, seg, c.n_scenes)];
   if (kSolve) {
-    const int2 nr = nrange[s];
     float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
-    for (int j = 0; j < nr.y; ++j) {
-      const int t = nbr[nr.x + j];
-      const float4 q = P[t];
-      const float dx = pi.x - q.x;
-      const float dy = pi.y - q.y;
-      const float dz = pi.z - q.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < c.h2) {
-        const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
-        const float f = spiky(r, c);
-        float sv = pi.w + q.w;
-        if (c.scorr_on) {
-          const float W = poly6(r2, c);
-          const float ratio = W * c.scorr_inv_wdq;
-          const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
-          sv += scorr;
-        }
-        ddx += sv * f * dx;
-        ddy += sv * f * dy;
-        ddz += sv * f * dz;
-      }
-    }
+    for_each_neighbor(
+        nv, s, [&](int t) { return P[t]; },
+        [&](int, const float4& q) {
+          const float dx = pi.x - q.x;
+          const float dy = pi.y - q.y;
+          const float dz = pi.z - q.z;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < c.h2) {
+            const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+            const float f = same ? c.f_min : pair_spiky(r2, c);
+            float sv = pi.w + q.w;
+            if (c.scorr_on) {
+              const float W = same ? c.w0 : poly6(r2, c);
+              const float ratio = W * c.scorr_inv_wdq;
+              const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
+              sv += scorr;
+            }
+            const float sf = sv * f;
+            if (same && isfinite(sf)) return;  // (sv f) * (+-0): +-0
+            ddx += sf * dx;
+            ddy += sf * dy;
+            ddz += sf * dz;
+          }
+        });
     ddx *= c.inv_density;
     ddy *= c.inv_density;
     ddz *= c.inv_density;
@@ -632,34 +779,31 @@ __global__ __launch_bounds__(1024) void k_delta(
 template <bool kRest>
 __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __restrict__ pos,
                                                const float4* __restrict__ vin,
-                                               float4* __restrict__ vout,
-                                               const int2* __restrict__ nrange,
-                                               const int* __restrict__ nbr,
+                                               float4* __restrict__ vout, NbrView nv,
                                                const int* __restrict__ seg,
                                                const ScenePlanes* __restrict__ sp,
                                                const unsigned* __restrict__ flags) {
   const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (s >= c.n || fatal(flags)) return;
-  const int2 nr = nrange[s];
   const float3 xi = ld3(pos, s);
   const float4 vi = vin[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int j = 0; j < nr.y; ++j) {
-    const int t = nbr[nr.x + j];
-    const float3 xj = ld3(pos, t);
-    const float dx = xi.x - xj.x;
-    const float dy = xi.y - xj.y;
-    const float dz = xi.z - xj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < c.h2) {
-      const float4 vj = vin[t];
-      const float W = poly6(r2, c);
-      const float irho = (vj.w > 0.0f) ? (c.mass / vj.w) : 0.0f;
-      ax += (vj.x - vi.x) * W * irho;
-      ay += (vj.y - vi.y) * W * irho;
-      az += (vj.z - vi.z) * W * irho;
-    }
-  }
+  for_each_neighbor(
+      nv, s, [&](int t) { return Pair2{pos[t], vin[t]}; },
+      [&](int, const Pair2& q) {
+        const float dx = xi.x - q.a.x;
+        const float dy = xi.y - q.a.y;
+        const float dz = xi.z - q.a.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < c.h2) {
+          const float4 vj = q.b;
+          const float W = poly6(r2, c);
+          const float irho = (vj.w > 0.0f) ? (c.mass / vj.w) : 0.0f;
+          ax += (vj.x - vi.x) * W * irho;
+          ay += (vj.y - vi.y) * W * irho;
+          az += (vj.z - vi.z) * W * irho;
+        }
+      });
   float vx = vi.x + c.visc_c * ax;
   float vy = vi.y + c.visc_c * ay;
   float vz = vi.z + c.visc_c * az;
@@ -670,34 +814,32 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
 // vorticity: omega and |omega| (core.cpp:472-508)
 __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4* __restrict__ pos,
                                                      const float4* __restrict__ vel,
-                                                     float4* __restrict__ omega,
-                                                     const int2* __restrict__ nrange,
-                                                     const int* __restrict__ nbr,
+                                                     float4* __restrict__ omega, NbrView nv,
                                                      const unsigned* __restrict__ flags) {
   const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (s >= c.n || fatal(flags)) return;
-  const int2 nr = nrange[s];
   const float3 xi = ld3(pos, s);
   const float3 vi = ld3(vel, s);
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  for (int j = 0; j < nr.y; ++j) {
-    const int t = nbr[nr.x + j];
-    const float3 xj = ld3(pos, t);
-    const float dx = xi.x - xj.x;
-    const float dy = xi.y - xj.y;
-    const float dz = xi.z - xj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < c.h2) {
-      const float3 vj = ld3(vel, t);
-      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
-      const float f = spiky(r, c);
-      const float gx = f * dx, gy = f * dy, gz = f * dz;
-      const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-      ox += dvy * gz - dvz * gy;
-      oy += dvz * gx - dvx * gz;
-      oz += dvx * gy - dvy * gx;
-    }
-  }
+  for_each_neighbor(
+      nv, s, [&](int t) { return Pair2{pos[t], vel[t]}; },
+      [&](int, const Pair2& q) {
+        const float dx = xi.x - q.a.x;
+        const float dy = xi.y - q.a.y;
+        const float dz = xi.z - q.a.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < c.h2) {
+          const float dvx = q.b.x - vi.x, dvy = q.b.y - vi.y, dvz = q.b.z - vi.z;
+          if (dx == 0.0f && dy == 0.0f && dz == 0.0f && isfinite(dvx) && isfinite(dvy) &&
+              isfinite(dvz))
+            return;  // dv x (f * (+-0)) is a signed zero
+          const float f = pair_spiky(r2, c);
+          const float gx = f * dx, gy = f * dy, gz = f * dz;
+          ox += dvy * gz - dvz * gy;
+          oy += dvz * gx - dvx * gz;
+          oz += dvx * gy - dvy * gx;
+        }
+      });
   omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
 }
 
@@ -706,36 +848,36 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
 template <bool kRest>
 __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4* __restrict__ pos,
                                                      const float4* __restrict__ omega,
-                                                     float4* __restrict__ vel,
-                                                     const int2* __restrict__ nrange,
-                                                     const int* __restrict__ nbr,
+                                                     float4* __restrict__ vel, NbrView nv,
                                                      const int* __restrict__ seg,
                                                      const ScenePlanes* __restrict__ sp,
                                                      const unsigned* __restrict__ flags) {
   const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (s >= c.n || fatal(flags)) return;
-  const int2 nr = nrange[s];
   const float3 xi = ld3(pos, s);
   const float4 om = omega[s];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  for (int j = 0; j < nr.y; ++j) {
-    const int t = nbr[nr.x + j];
-    const float3 xj = ld3(pos, t);
-    const float dx = xi.x - xj.x;
-    const float dy = xi.y - xj.y;
-    const float dz = xi.z - xj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < c.h2) {
-      const float omj = reinterpret_cast<const float*>(omega + t)[3];
-      const float r = sqrtf(r2 < c.min_r2 ? c.min_r2 : r2);
-      const float f = spiky(r, c);
-      const float gx = f * dx, gy = f * dy, gz = f * dz;
-      const float co = omj - om.w;
-      ex += co * gx;
-      ey += co * gy;
-      ez += co * gz;
-    }
-  }
+  for_each_neighbor(
+      nv, s,
+      [&](int t) {
+        const float3 x = ld3(pos, t);
+        return make_float4(x.x, x.y, x.z, reinterpret_cast<const float*>(omega + t)[3]);
+      },
+      [&](int, const float4& q) {
+        const float dx = xi.x - q.x;
+        const float dy = xi.y - q.y;
+        const float dz = xi.z - q.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < c.h2) {
+          const float co = q.w - om.w;
+          if (dx == 0.0f && dy == 0.0f && dz == 0.0f && isfinite(co)) return;
+          const float f = pair_spiky(r2, c);
+          const float gx = f * dx, gy = f * dy, gz = f * dz;
+          ex += co * gx;
+          ey += co * gy;
+          ez += co * gz;
+        }
+      });
   const float len = sqrtf(ex * ex + ey * ey + ez * ez);
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
   if (len > c.vort_norm_eps) {
@@ -828,10 +970,11 @@ hipError_t enqueue_substep(Ctx& c) {
 
   FH_TRY(L("predict_bbox", [&] {
     k_predict_bbox<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.planes, c.seg, c.cellc, c.bbox,
-                                      c.flags);
+                                      c.tile_bbox, c.flags);
   }));
   FH_TRY(L("grid_setup", [&] {
-    k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.bbox, c.grid, c.total_cells, c.cell_cap,
+    k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.seg, c.tile_bbox, bs, c.bbox, c.grid,
+                                    c.total_cells, c.cell_cap,
                                     c.flags, c.nbr_total, c.need_cells, c.steps_started);
   }));
   FH_TRY(L("cell_count", [&] {
@@ -855,12 +998,13 @@ hipError_t enqueue_substep(Ctx& c) {
                                  c.diag_key, c.flags);
   }));
   FH_TRY(L("neighbors", [&] {
-    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], c.seg, c.grid, c.start, c.nrange, c.nbr,
-                                   c.nbr_cap, c.nbr_total, c.flags);
+    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], c.seg, c.grid, c.start, c.noff, c.nwords,
+                                   c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
   }));
 
   // velocity buffer that the commit writes: XSPH needs a second buffer to
   // stay Jacobi, so the commit goes to velB and XSPH back to velA.
+  const NbrView nv{c.noff, c.nwords, c.nbr};
   float4* v1 = c.xsph ? c.velB : c.velA;
   const bool rest_in_delta = c.restitution_any && !c.xsph && !c.vort;
   const int I = c.iterations;
@@ -869,19 +1013,19 @@ hipError_t enqueue_substep(Ctx& c) {
     float4* Pn = c.pred[(it + 1) & 1];
     const bool last = (it + 1 == I);
     FH_TRY(L("lambda", [&] {
-      k_lambda<<<nb, bs, 0, st>>>(k, P, c.nrange, c.nbr, c.rho, last ? 1 : 0, c.flags);
+      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, c.rho, last ? 1 : 0, c.flags);
     }));
     FH_TRY(L("delta", [&] {
       if (!last)
-        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
                                                        c.flags);
       else if (rest_in_delta)
-        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
                                                      c.planes, c.posB, c.posA, v1, c.rho,
                                                      c.flags);
       else
-        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, c.nrange, c.nbr, c.seg,
+        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
                                                       c.flags);
     }));
@@ -889,12 +1033,11 @@ hipError_t enqueue_substep(Ctx& c) {
   if (I == 0) {
     FH_TRY(L("delta", [&] {
       if (rest_in_delta)
-        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], c.nrange, c.nbr,
+        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv,
                                                       c.seg, c.planes, c.posB, c.posA, v1,
                                                       c.rho, c.flags);
       else
-        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], c.nrange,
-                                                       c.nbr, c.seg, c.planes, c.posB, c.posA,
+        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, c.seg, c.planes, c.posB, c.posA,
                                                        v1, c.rho, c.flags);
     }));
   }
@@ -902,23 +1045,23 @@ hipError_t enqueue_substep(Ctx& c) {
     const bool rest = c.restitution_any && !c.vort;
     FH_TRY(L("xsph", [&] {
       if (rest)
-        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, c.nrange, c.nbr, c.seg,
+        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, c.seg,
                                         c.planes, c.flags);
       else
-        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, c.nrange, c.nbr, c.seg,
+        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, c.seg,
                                          c.planes, c.flags);
     }));
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
-      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, c.nrange, c.nbr, c.flags);
+      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, c.flags);
     }));
     FH_TRY(L("vorticity_apply", [&] {
       if (c.restitution_any)
-        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, c.nrange, c.nbr,
+        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv,
                                               c.seg, c.planes, c.flags);
       else
-        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, c.nrange, c.nbr,
+        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv,
                                                c.seg, c.planes, c.flags);
     }));
   }

@@ -73,6 +73,7 @@ _SIGS = {
     "fluid_hip_get_neighbors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64,
                                           C.POINTER(C.c_int64)]),
     "fluid_hip_get_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fluid_hip_get_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
     "fluid_hip_profile": (C.c_int, [C.c_void_p, C.c_int32]),
     "fluid_hip_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double),
                                          C.POINTER(C.c_int64)]),
@@ -257,6 +258,14 @@ class HipSolver:
         _check(self.lib.fluid_hip_get_grid(self.h, keys.ctypes.data if keys is not None else None,
                                            desc.ctypes.data))
         return keys, desc
+
+    def stats(self):
+        """Sizes of the last step: neighbor pairs, encoded list words, grid
+        cells, list capacity (words), cell capacity."""
+        out = (C.c_int64 * 5)()
+        _check(self.lib.fluid_hip_get_stats(self.h, out, 5))
+        keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity")
+        return dict(zip(keys, [int(x) for x in out]))
 
     def profile(self, enable=True):
         _check(self.lib.fluid_hip_profile(self.h, int(bool(enable))))

@@ -165,6 +165,11 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* ids, int64_t cap, int64
  * int32 per scene: origin x,y,z, bricks x,y,z, cell offset, cell count. */
 int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
 
+/* Sizes of the last step: out[0] neighbor pairs (decoded), out[1] encoded
+ * list words, out[2] grid cells (all scenes), out[3] neighbor-list capacity
+ * (words), out[4] cell-table capacity.  `n` entries are written (<= 5). */
+int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
+
 /* ---- timing -------------------------------------------------------------- */
 
 /* Per-kernel device time (HIP events on the context's stream) of the steps
