@@ -140,6 +140,15 @@ void compute_consts(Ctx& c) {
   const float wdq = scorr_enabled ? host_poly6(dq * dq, h) : 0.0f;
   const float inv_wdq = (wdq > 1e-12f) ? (1.0f / wdq) : 0.0f;
   k.scorr_on = (scorr_enabled && inv_wdq > 0.0f) ? 1 : 0;
+  {
+    // (li + lj + scorr(r=0)) * f_min stays finite when |l| < lam_safe
+    const double ratio = static_cast<double>(k.w0) * inv_wdq;
+    const double sc = k.scorr_on ? std::fabs(p.scorr_k) * 2.0 * std::pow(ratio, p.scorr_n) : 0.0;
+    const double fm = std::fabs(static_cast<double>(k.f_min));
+    double lim = 0.5 * (3.4e38 / 8.0 / (fm > 0.0 ? fm : 1.0) - sc);
+    if (!(lim > 0.0)) lim = 0.0;
+    k.lam_safe = static_cast<float>(std::min(lim, 1.0e30));
+  }
   k.scorr_inv_wdq = inv_wdq;
   k.scorr_neg_k = -p.scorr_k;
   k.scorr_n = p.scorr_n;
@@ -148,6 +157,8 @@ void compute_consts(Ctx& c) {
   k.vort_norm_eps = p.vort_norm_eps;
   k.record_diag = c.record_diag ? 1 : 0;
   k.xcd_remap = c.xcd_remap;
+  k.coop = c.wave_mode >= 2 ? 1 : 0;
+  k.coop_bulk = c.wave_mode == 2 ? 1 : 0;
   c.consts = k;
   c.iterations = p.solver_iterations;
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423
@@ -204,11 +215,11 @@ void free_all(Ctx& c) {
   dfree(c.rho);
   dfree(c.cellc);
   for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
-                  &c.total_cells, &c.scan_partial, &c.diag_key, &c.nbr, &c.seg, &c.nwords,
+                  &c.total_cells, &c.scan_partial, &c.slot_key, &c.nbr, &c.seg, &c.nwords,
                   &c.ncount, &c.tile_bbox,
                   &c.steps_started})
     dfree(*p);
-  dfree(c.diag_cell);
+  dfree(c.slot_cell);
   dfree(c.grid);
   dfree(c.noff);
   dfree(c.nwords);
@@ -371,6 +382,9 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   c.block = bs;
   c.xcd_remap = cfg.xcd_remap ? 1 : 0;
   c.record_diag = cfg.record_diag != 0;
+  if (cfg.wave_mode < 0 || cfg.wave_mode > 3)
+    return fail(FLUID_HIP_EINVAL, "wave_mode must be 0..3");
+  c.wave_mode = cfg.wave_mode ? cfg.wave_mode : 2;
   c.n = static_cast<int>(total);
   c.n_scenes = n_scenes;
   c.params.assign(params, params + n_scenes);
@@ -412,10 +426,8 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.perm, n);
   ALLOC(c.sid, n);
   ALLOC(c.skey, n);
-  if (c.record_diag) {
-    ALLOC(c.diag_cell, n);
-    ALLOC(c.diag_key, n);
-  }
+  ALLOC(c.slot_cell, n);
+  ALLOC(c.slot_key, n);
   ALLOC(c.noff, n);
   ALLOC(c.nwords, n);
   ALLOC(c.ncount, n);
@@ -627,7 +639,7 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
   if (sorted_ids) std::copy(ids.begin(), ids.end(), sorted_ids);
   if (cell) {
     std::vector<uint64_t> dc(n);
-    HIP_TRY(hipMemcpy(dc.data(), c.diag_cell, sizeof(uint64_t) * n, hipMemcpyDeviceToHost),
+    HIP_TRY(hipMemcpy(dc.data(), c.slot_cell, sizeof(uint64_t) * n, hipMemcpyDeviceToHost),
             "read cells");
     const uint64_t m = (1ull << 21) - 1ull;
     for (size_t s = 0; s < n; ++s) {
@@ -711,8 +723,7 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc) {
   Ctx& c = ctx->c;
   if (!c.stepped) return fail(FLUID_HIP_ESTATE, "no step taken since upload");
   if (slot_keys) {
-    if (!c.record_diag) return fail(FLUID_HIP_ESTATE, "slot keys need record_diag=1");
-    HIP_TRY(hipMemcpy(slot_keys, c.diag_key, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read keys");
+    HIP_TRY(hipMemcpy(slot_keys, c.slot_key, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read keys");
   }
   if (desc) {
     static_assert(sizeof(fluidhip::GridDesc) == 8 * sizeof(int32_t), "GridDesc layout");

@@ -43,6 +43,7 @@ struct Ctx {
   int xcd_remap = 1;
   int scan_blocks = 512;
   bool record_diag = false;
+  int wave_mode = 2;
 
   // state (A = committed state between substeps)
   float4* posA = nullptr;  // xyz, w = original id bits
@@ -68,8 +69,8 @@ struct Ctx {
   GridDesc* grid = nullptr;
   int* total_cells = nullptr;
   int* scan_partial = nullptr;  // scan_blocks + 1
-  uint64_t* diag_cell = nullptr;
-  int* diag_key = nullptr;
+  uint64_t* slot_cell = nullptr;  // packed cell of pred0 per sorted slot
+  int* slot_key = nullptr;  // table key of each sorted slot's cell
 
   // neighbors
   long long* noff = nullptr;  // first word of each slot's list

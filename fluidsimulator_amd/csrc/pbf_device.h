@@ -33,6 +33,7 @@ struct StepConsts {
   float h2;         // h * h               (core.cpp:140)
   float min_r2;     // (0.01f*h)^2         (core.cpp:141-142)
   float f_min;      // spiky(sqrt(min_r2)): the factor of every clamped pair
+  float lam_safe;   // |lambda| below which (li + lj + scorr) * f_min is finite
   float w_coeff;    // 315 / (64 pi h^9)   (core.cpp:44)
   float w0;         // poly6(0, h)         (core.cpp:319)
   float s_coeff;    // -45 / (pi h^6)      (core.cpp:55)
@@ -49,6 +50,8 @@ struct StepConsts {
   float vort_norm_eps;
   int record_diag;
   int xcd_remap;
+  int coop;         // 1: wavefronts sharing a cell use shared candidate tiles
+  int coop_bulk;    // 1: ... and whole coincident tiles in one step
 };
 
 struct ScenePlanes {

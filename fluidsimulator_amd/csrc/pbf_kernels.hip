@@ -37,6 +37,8 @@ namespace fluidhip {
 namespace {
 
 constexpr int kWave = 64;
+// Bound below which a difference of two floats cannot overflow.
+constexpr float kHalfMax = 1.0e38f;
 
 // Kernels launched after the grid setup stop as soon as an earlier phase has
 // raised a fatal flag; the committed state is then left untouched.
@@ -125,6 +127,22 @@ __device__ __forceinline__ void restitute(float& vx, float& vy, float& vz, float
       vz = tz * scale + vn_new * pl.z;
     }
   }
+}
+
+// ---------------------------------------------------------------- wavefront
+
+// Lane-to-lane broadcasts.  They must run on the whole wavefront: the
+// value of lane j is only there if lane j executed the code producing it,
+// so no broadcast sits under a lane-dependent branch.
+__device__ __forceinline__ int bcast(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ float3 bcast(const float3& v, int j) {
+  return make_float3(bcast(v.x, j), bcast(v.y, j), bcast(v.z, j));
+}
+__device__ __forceinline__ float4 bcast(const float4& v, int j) {
+  return make_float4(bcast(v.x, j), bcast(v.y, j), bcast(v.z, j), bcast(v.w, j));
 }
 
 // ---------------------------------------------------------------- transfers
@@ -443,6 +461,32 @@ __global__ __launch_bounds__(1024) void k_scatter(StepConsts c, const int* __res
   skey[slot] = k;
 }
 
+// Lanes whose own cell holds at least this many particles are served by
+// wavefront-cooperative walks (one per distinct dense cell in the
+// wavefront); the others walk alone.
+constexpr int kDenseCell = 32;
+// ... and so are lanes with at least this many neighbors (or, while the
+// lists are built, candidates).
+constexpr int kHeavyList = 256;
+
+// Calls fn(leader, mine) once per distinct key among the lanes with
+// want == true: `leader` is the lowest such lane holding the key and `mine`
+// marks the lanes that share it.  Wavefront-uniform control flow; every
+// lane must be present.
+template <typename Fn>
+__device__ __forceinline__ void for_each_key_group(bool want, int key, Fn&& fn) {
+  bool pend = want;
+  while (true) {
+    const unsigned long long m = __ballot(pend);
+    if (m == 0ull) break;
+    const int leader = __builtin_ctzll(m);
+    const int kl = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = pend && key == kl;
+    fn(leader, mine);
+    pend = pend && !mine;
+  }
+}
+
 // Rank by original id inside the cell, then gather the particle into its
 // final slot and recompute pred0 there (same float ops as predict_bbox).
 __global__ __launch_bounds__(1024) void k_reorder(
@@ -451,17 +495,42 @@ __global__ __launch_bounds__(1024) void k_reorder(
     const ScenePlanes* __restrict__ sp, const float4* __restrict__ posA,
     const float4* __restrict__ velA, float4* __restrict__ posB, float4* __restrict__ velB,
     float4* __restrict__ pred, const uint64_t* __restrict__ cellc,
-    uint64_t* __restrict__ diag_cell, int* __restrict__ diag_key,
+    uint64_t* __restrict__ slot_cell, int* __restrict__ slot_key,
     const unsigned* __restrict__ flags) {
-  const int u = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (u >= c.n || fatal(flags)) return;
+  const int u0 = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (fatal(flags)) return;
+  // lanes past the end stand in for the last particle so that the
+  // wavefront stays whole for the cooperative ranks below
+  const bool active = u0 < c.n;
+  const int u = active ? u0 : c.n - 1;
   const int p = perm[u];
   const int id = sid[u];
   const int k = skey[u];
   const int a = start[k];
   const int b = start[k + 1];
   int r = 0;
-  for (int t = a; t < b; ++t) r += (sid[t] < id) ? 1 : 0;
+  const bool dense = c.coop && (b - a) >= kDenseCell;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (c.coop) {
+    for_each_key_group(dense, k, [&](int leader, bool mine) {
+      // the group's cell: shared, coalesced id tiles
+      const int au = __builtin_amdgcn_readlane(a, leader);
+      const int bu = __builtin_amdgcn_readlane(b, leader);
+      for (int base = au; base < bu; base += kWave) {
+        const int m = min(kWave, bu - base);
+        const int tile = sid[base + min(lane, m - 1)];
+        // broadcasts run on the whole wavefront (a load whose only use sat
+        // under `mine` could be moved there, leaving other lanes' values
+        // unloaded); only the count is predicated
+        for (int j = 0; j < m; ++j) {
+          const int tj = bcast(tile, j);
+          r += (mine && tj < id) ? 1 : 0;
+        }
+      }
+    });
+  }
+  if (!dense)
+    for (int t = a; t < b; ++t) r += (sid[t] < id) ? 1 : 0;
   const int dest = a + r;
   const ScenePlanes& s = sp[scene_of(u, seg, c.n_scenes)];
   const float4 x = posA[p];
@@ -469,13 +538,12 @@ __global__ __launch_bounds__(1024) void k_reorder(
   const float vx = v.x + s.gx * c.dt;
   const float vy = v.y + s.gy * c.dt;
   const float vz = v.z + s.gz * c.dt;
+  if (!active) return;
   posB[dest] = x;
   velB[dest] = make_float4(vx, vy, vz, 0.0f);
   pred[dest] = make_float4(x.x + vx * c.dt, x.y + vy * c.dt, x.z + vz * c.dt, 0.0f);
-  if (c.record_diag) {
-    diag_cell[dest] = cellc[p];
-    diag_key[dest] = k;
-  }
+  slot_cell[dest] = cellc[p];
+  slot_key[dest] = k;
 }
 
 // ---------------------------------------------------------------- neighbors
@@ -517,8 +585,23 @@ struct RunWriter {
     run_start = t;
     run_len = 1;
   }
+  // add(a), add(a+1), ..., add(b-1) in O(1)
+  template <bool kFill>
+  __device__ __forceinline__ void add_range(int a, int b) {
+    if (a >= b) return;
+    count += b - a;
+    if (run_len > 0 && a == run_start + run_len) {
+      run_len += b - a;
+      return;
+    }
+    flush<kFill>();
+    run_start = a;
+    run_len = b - a;
+  }
 };
 
+// Candidates of a particle: the 27 cells around its own in (dz, dy, dx)
+// order, each cell's slots in ascending order, i.e. ascending original id.
 template <bool kFill>
 __device__ __forceinline__ void walk_cells(int s, float px, float py, float pz, int rx, int ry,
                                           int rz, const GridDesc& g, const StepConsts& c,
@@ -543,12 +626,150 @@ __device__ __forceinline__ void walk_cells(int s, float px, float py, float pz, 
   w.flush<kFill>();
 }
 
+// Per-slot cell data the cooperative walks need.
+struct CellView {
+  const uint64_t* __restrict__ slot_cell;  // packed absolute cell
+  const int* __restrict__ slot_key;        // table key of that cell
+  const GridDesc* __restrict__ grid;
+  const int* __restrict__ start;
+  const int* __restrict__ seg;
+};
+
+// One lane's view: its slot (the last particle for lanes past the end, so
+// that every wavefront stays whole), cell and whether that cell is dense.
+struct LaneCell {
+  int s;
+  bool active;
+  bool dense;
+  int key;
+  uint64_t cell;
+};
+
+// `heavy` adds lanes of sparse cells whose own walk is long (next to a
+// dense cell): they are grouped by their cell too.
+__device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellView& cv, int s0,
+                                              const int* __restrict__ ncount = nullptr) {
+  LaneCell L;
+  L.active = s0 < c.n;
+  L.s = L.active ? s0 : c.n - 1;
+  L.key = cv.slot_key[L.s];
+  L.cell = cv.slot_cell[L.s];
+  const bool heavy = ncount != nullptr && ncount[L.s] >= kHeavyList;
+  L.dense = c.coop && L.active &&
+            (heavy || (cv.start[L.key + 1] - cv.start[L.key]) >= kDenseCell);
+  return L;
+}
+
+// Wavefront-uniform description of the group led by `leader`.
+struct Group {
+  int rx, ry, rz;
+  GridDesc g;
+};
+
+__device__ __forceinline__ Group group_of(const StepConsts& c, const CellView& cv,
+                                          const LaneCell& L, int leader) {
+  Group gr;
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(L.cell)), leader);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(L.cell >> 32)),
+                                           leader);
+  int sc = 0;
+  if (c.n_scenes > 1) {
+    const int sl = __builtin_amdgcn_readlane(L.s, leader);
+    sc = __builtin_amdgcn_readfirstlane(scene_of(sl, cv.seg, c.n_scenes));
+  }
+  gr.g = cv.grid[sc];
+  int x, y, z;
+  unpack_cell((static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo),
+              x, y, z);
+  gr.rx = x - gr.g.ox;
+  gr.ry = y - gr.g.oy;
+  gr.rz = z - gr.g.oz;
+  return gr;
+}
+
+// Calls body(base, m, tile) for each run of up to 64 candidates of the
+// group's 27-cell neighborhood, in walk order; tile holds load(candidate)
+// of lane `lane` (lanes past m repeat the last candidate).
+template <typename Load, typename Body>
+__device__ __forceinline__ void group_tiles(const Group& gr, const int* __restrict__ start,
+                                            Load&& load, Body&& body) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int k = __builtin_amdgcn_readfirstlane(
+            cell_key(gr.rx + dx, gr.ry + dy, gr.rz + dz, gr.g));
+        const int a = __builtin_amdgcn_readfirstlane(start[k]);
+        const int b = __builtin_amdgcn_readfirstlane(start[k + 1]);
+        for (int base = a; base < b; base += kWave) {
+          const int m = min(kWave, b - base);
+          const auto tile = load(base + min(lane, m - 1));
+          body(base, m, tile);
+        }
+      }
+}
+
+// True when the m candidates of a tile share one position (bitwise), which
+// is returned in cp; wavefront-uniform.
+template <typename T>
+__device__ __forceinline__ bool tile_one_point(const T& q, int m, float3& cp) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int x0 = __builtin_amdgcn_readlane(__float_as_int(q.x), 0);
+  const int y0 = __builtin_amdgcn_readlane(__float_as_int(q.y), 0);
+  const int z0 = __builtin_amdgcn_readlane(__float_as_int(q.z), 0);
+  cp = make_float3(__int_as_float(x0), __int_as_float(y0), __int_as_float(z0));
+  const bool eq = lane >= m || (__float_as_int(q.x) == x0 && __float_as_int(q.y) == y0 &&
+                                __float_as_int(q.z) == z0);
+  return __ballot(eq) == ~0ull;
+}
+
+__device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
+__device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
+
+template <bool kFill>
+__device__ __forceinline__ void group_walk_cells(const Group& gr, bool mine, int s, float px,
+                                                float py, float pz, const StepConsts& c,
+                                                const int* __restrict__ start,
+                                                const float4* __restrict__ pred, RunWriter& w) {
+  group_tiles(gr, start, [&](int t) { return ld3(pred, t); },
+              [&](int base, int m, const float3& tile) {
+                float3 cp;
+                if (c.coop_bulk && tile_one_point(tile, m, cp)) {
+                  // one test decides the whole tile (all but the lane itself)
+                  if (mine) {
+                    const float ex = px - cp.x;
+                    const float ey = py - cp.y;
+                    const float ez = pz - cp.z;
+                    const float r2 = ex * ex + ey * ey + ez * ez;
+                    if (r2 < c.h2) {
+                      if (s >= base && s < base + m) {
+                        w.add_range<kFill>(base, s);
+                        w.add_range<kFill>(s + 1, base + m);
+                      } else {
+                        w.add_range<kFill>(base, base + m);
+                      }
+                    }
+                  }
+                  return;
+                }
+                for (int j = 0; j < m; ++j) {
+                  const float3 q = bcast(tile, j);  // whole wavefront
+                  const int t = base + j;
+                  if (!mine || t == s) continue;
+                  const float ex = px - q.x;
+                  const float ey = py - q.y;
+                  const float ez = pz - q.z;
+                  const float r2 = ex * ex + ey * ey + ez * ez;
+                  if (r2 < c.h2) w.add<kFill>(t);
+                }
+              });
+  if (mine) w.flush<kFill>();
+}
+
 __global__ __launch_bounds__(1024) void k_neighbors(
-    StepConsts c, const float4* __restrict__ pred, const int* __restrict__ seg,
-    const GridDesc* __restrict__ grid, const int* __restrict__ start,
-    long long* __restrict__ noff, int* __restrict__ nwords, int* __restrict__ ncount,
-    int* __restrict__ nbr, long long nbr_cap, unsigned long long* __restrict__ nbr_total,
-    unsigned* __restrict__ flags) {
+    StepConsts c, const float4* __restrict__ pred, CellView cv, long long* __restrict__ noff,
+    int* __restrict__ nwords, int* __restrict__ ncount, int* __restrict__ nbr, long long nbr_cap,
+    unsigned long long* __restrict__ nbr_total, unsigned* __restrict__ flags) {
   __shared__ int lds[1024 / kWave + 1];
   __shared__ long long base_sh;
   __shared__ int abort_sh;
@@ -557,65 +778,74 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   if (threadIdx.x == 0) abort_sh = fatal(flags) ? 1 : 0;
   __syncthreads();
   if (abort_sh) return;
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  bool active = s < c.n;
-  float px = 0.f, py = 0.f, pz = 0.f;
-  int rx = 0, ry = 0, rz = 0;
-  GridDesc g{};
-  RunWriter w{nullptr};
-  if (active) {
-    const float3 p = ld3(pred, s);
-    px = p.x, py = p.y, pz = p.z;
-    g = grid[scene_of(s, seg, c.n_scenes)];
-    bool bad = false;
-    rx = floor_cell(px, c.inv_h, bad) - g.ox;
-    ry = floor_cell(py, c.inv_h, bad) - g.oy;
-    rz = floor_cell(pz, c.inv_h, bad) - g.oz;
-    // The table has a one-cell margin around the bounding box of these same
-    // cells; anything else would index outside it, so refuse instead.
-    if (bad || rx < 1 || ry < 1 || rz < 1 || rx > 4 * g.nbx - 2 || ry > 4 * g.nby - 2 ||
-        rz > 4 * g.nbz - 2) {
-      atomicOr(flags, kFlagRange);
-      active = false;
-    } else {
-      walk_cells<false>(s, px, py, pz, rx, ry, rz, g, c, start, pred, w);
-    }
+  const LaneCell L = lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x);
+  const int s = L.s;
+  bool active = L.active;
+  const float3 p = ld3(pred, s);
+  const GridDesc g = cv.grid[scene_of(s, cv.seg, c.n_scenes)];
+  bool bad = false;
+  const int rx = floor_cell(p.x, c.inv_h, bad) - g.ox;
+  const int ry = floor_cell(p.y, c.inv_h, bad) - g.oy;
+  const int rz = floor_cell(p.z, c.inv_h, bad) - g.oz;
+  // The table has a one-cell margin around the bounding box of these same
+  // cells; anything else would index outside it, so refuse instead.
+  if (active && (bad || rx < 1 || ry < 1 || rz < 1 || rx > 4 * g.nbx - 2 || ry > 4 * g.nby - 2 ||
+                 rz > 4 * g.nbz - 2)) {
+    atomicOr(flags, kFlagRange);
+    active = false;
   }
+  // lanes of sparse cells next to dense ones: long candidate walks
+  bool heavy = false;
+  if (c.coop && active && !L.dense) {
+    int cand = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int k = cell_key(rx + dx, ry + dy, rz + dz, g);
+          cand += cv.start[k + 1] - cv.start[k];
+        }
+    heavy = cand >= kHeavyList;
+  }
+  // every dense group lies inside the table: its cell is some valid lane's
+  const bool dense = (L.dense || heavy) && active;
+  RunWriter w{nullptr};
+  if (c.coop)
+    for_each_key_group(dense, L.key, [&](int leader, bool mine) {
+      group_walk_cells<false>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
+                              pred, w);
+    });
+  if (active && !dense) walk_cells<false>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, w);
   // The encoded length of one list stays below 2^31 words (at most two words
   // per neighbor and fewer than 2^29 particles).
-  const int nw = static_cast<int>(w.words);
+  const int nw = active ? static_cast<int>(w.words) : 0;
   int total;
   const int pre = block_excl_scan(nw, lds, &total);
   if (threadIdx.x == 0)
     base_sh = static_cast<long long>(atomicAdd(nbr_total, static_cast<unsigned long long>(total)));
   __syncthreads();
-  if (!active) {
-    if (s < c.n) {
-      noff[s] = 0;
-      nwords[s] = 0;
-      ncount[s] = 0;
-    }
-    return;
-  }
   const long long off = base_sh + pre;
-  if (off + nw > nbr_cap) {
-    atomicOr(flags, kFlagNeighbors);
-    noff[s] = 0;
-    nwords[s] = 0;
-    ncount[s] = 0;
-    return;
+  const bool fits = off + nw <= nbr_cap;
+  if (active && !fits) atomicOr(flags, kFlagNeighbors);
+  const bool ok = active && fits;
+  RunWriter f{nbr + (ok ? off : 0)};
+  if (c.coop)
+    for_each_key_group(dense && ok, L.key, [&](int leader, bool mine) {
+      group_walk_cells<true>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
+                             pred, f);
+    });
+  if (ok && !dense) walk_cells<true>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, f);
+  if (L.active) {
+    noff[s] = ok ? off : 0;
+    nwords[s] = ok ? nw : 0;
+    ncount[s] = ok ? f.count : 0;
   }
-  RunWriter f{nbr + off};
-  walk_cells<true>(s, px, py, pz, rx, ry, rz, g, c, start, pred, f);
-  noff[s] = off;
-  nwords[s] = nw;
-  ncount[s] = f.count;
 }
 
 struct NbrView {
   const long long* __restrict__ off;
   const int* __restrict__ nw;
   const int* __restrict__ list;
+  const int* __restrict__ count;
 };
 
 // Calls f(t, load(t)) for each neighbor slot t of slot s, in list order.
@@ -646,6 +876,96 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& v, int s, Load&
   }
 }
 
+// A lane's position in its own list during a cooperative walk.
+struct ListCursor {
+  const int* __restrict__ L;
+  int nw, w, cur, end;
+  __device__ __forceinline__ void advance() {
+    if (w < nw) {
+      const int e = L[w++];
+      cur = e & 0x7fffffff;
+      end = cur + 1;
+      if (e < 0) end = cur + L[w++];
+    } else {
+      cur = INT_MAX;
+      end = INT_MAX;
+    }
+  }
+  // Consumes this lane's neighbors inside [base, base + m) and returns how
+  // many there were.  The cells are visited in (dz, dy, dx) order, not slot
+  // order: a run that starts below this tile belongs to a later cell.
+  __device__ __forceinline__ int take(int base, int m) {
+    const int hi = base + m;
+    int cnt = 0;
+    while (cur >= base && cur < hi) {
+      const int e2 = min(end, hi);
+      cnt += e2 - cur;
+      cur = e2;
+      if (cur == end) advance();
+    }
+    return cnt;
+  }
+  __device__ __forceinline__ bool hit(int t) {
+    if (t != cur) return false;
+    if (++cur == end) advance();
+    return true;
+  }
+};
+
+// Gathers over the neighbors of every lane of a wavefront, in each lane's
+// list order:
+//  * lanes in dense cells: one cooperative walk per distinct dense cell of
+//    the wavefront; candidate tiles are loaded once (coalesced) and handed
+//    lane to lane with readlane; each lane takes exactly the candidates its
+//    own list holds;
+//  * when every candidate of a tile sits at one point cp, prep(cp, flag)
+//    hoists the pair geometry once per lane and says how the lane's
+//    neighbors in the tile contribute: 0 one by one through pair_u,
+//    1 nothing at all (exact signed zeros), 2 `count` identical terms
+//    (on_count); flag is tflag(tile, m), evaluated by the whole wavefront;
+//  * other lanes walk their own run-length-encoded lists.
+template <typename Load, typename Pair, typename TFlag, typename Prep, typename PairU,
+          typename OnCount>
+__device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, const CellView& cv,
+                                       const LaneCell& L, Load&& load, Pair&& pair,
+                                       TFlag&& tflag, Prep&& prep, PairU&& pair_u,
+                                       OnCount&& on_count) {
+  if (c.coop)
+    for_each_key_group(L.dense, L.key, [&](int leader, bool mine) {
+      ListCursor lc{nv.list + nv.off[L.s], nv.nw[L.s], 0, INT_MAX, INT_MAX};
+      if (mine) lc.advance();
+      group_tiles(group_of(c, cv, L, leader), cv.start, load,
+                  [&](int base, int m, const auto& tile) {
+                    float3 cp;
+                    if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
+                      const bool tf = tflag(tile, m);
+                      const int mode = mine ? prep(cp, tf) : 1;
+                      if (__all(mode != 0)) {
+                        const int cnt = lc.take(base, m);
+                        if (mode == 2) on_count(cnt);
+                        return;
+                      }
+                      for (int j = 0; j < m; ++j) {
+                        const auto q = bcast(tile, j);
+                        if (lc.hit(base + j)) {
+                          if (mode == 0)
+                            pair_u(q);
+                          else if (mode == 2)
+                            on_count(1);
+                        }
+                      }
+                      return;
+                    }
+                    for (int j = 0; j < m; ++j) {
+                      const auto q = bcast(tile, j);
+                      if (lc.hit(base + j)) pair(q);
+                    }
+                  });
+    });
+  if (L.active && !L.dense)
+    for_each_neighbor(nv, L.s, load, [&](int, const auto& q) { pair(q); });
+}
+
 // Spiky factor of a pair at squared distance r2 < h2: pairs closer than
 // min_r share one precomputed factor (core.cpp:303-304 clamps r there), so
 // only the others pay for the IEEE square root.
@@ -654,9 +974,42 @@ __device__ __forceinline__ float pair_spiky(float r2, const StepConsts& c) {
   return spiky(sqrtf(r2), c);
 }
 
-struct Pair2 {
-  float4 a, b;
+// Candidate records loaded once per tile and broadcast lane to lane.
+struct EPos {
+  float3 p;
 };
+struct EPosL {
+  float3 p;
+  float l;  // lambda
+};
+struct EPosV {
+  float3 p;
+  float3 v;
+  float irho;  // m / rho (0 when rho <= 0)
+};
+struct EPosV3 {
+  float3 p;
+  float3 v;
+};
+struct EPosW {
+  float3 p;
+  float w;  // |omega|
+};
+__device__ __forceinline__ EPos bcast(const EPos& e, int j) { return EPos{bcast(e.p, j)}; }
+__device__ __forceinline__ EPosL bcast(const EPosL& e, int j) {
+  return EPosL{bcast(e.p, j), bcast(e.l, j)};
+}
+__device__ __forceinline__ EPosV bcast(const EPosV& e, int j) {
+  return EPosV{bcast(e.p, j), bcast(e.v, j), bcast(e.irho, j)};
+}
+__device__ __forceinline__ EPosV3 bcast(const EPosV3& e, int j) {
+  return EPosV3{bcast(e.p, j), bcast(e.v, j)};
+}
+__device__ __forceinline__ EPosW bcast(const EPosW& e, int j) {
+  return EPosW{bcast(e.p, j), bcast(e.w, j)};
+}
+
+__device__ __forceinline__ bool tile_all(bool ok) { return __ballot(ok) == ~0ull; }
 
 // ---------------------------------------------------------------- solver
 //
@@ -666,39 +1019,81 @@ struct Pair2 {
 // cancellation to +0), so adding a signed zero leaves them bit-for-bit
 // unchanged and those additions are skipped.  Only finite products are
 // skipped: a non-finite factor times zero is NaN and is added as the
-// reference adds it.
+// reference adds it.  A tile whose candidates all sit at one point gives
+// every lane one fixed pair geometry, computed once for the tile.
 
 // lambda pass (core.cpp:281-329).  Writes lambda into pred[s].w.
 __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restrict__ P,
-                                                 NbrView nv, float* __restrict__ rho_out,
-                                                 int write_rho, const unsigned* __restrict__ flags) {
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (s >= c.n || fatal(flags)) return;
+                                                 NbrView nv, CellView cv,
+                                                 float* __restrict__ rho_out, int write_rho,
+                                                 const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const LaneCell L =
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+  const int s = L.s;
   const float3 p = ld3(P, s);
   float rho = 0.0f, gsx = 0.0f, gsy = 0.0f, gsz = 0.0f, sg2 = 0.0f;
   const float ngs = -c.grad_scale;
-  for_each_neighbor(
-      nv, s, [&](int t) { return ld3(P, t); },
-      [&](int, const float3& q) {
-        const float dx = p.x - q.x;
-        const float dy = p.y - q.y;
-        const float dz = p.z - q.z;
+  // per-pair increments of a uniform tile
+  float iw = 0.0f, igx = 0.0f, igy = 0.0f, igz = 0.0f, ij2 = 0.0f;
+  bool igrad = false;
+  auto pair = [&](const EPos& q) {
+    const float dx = p.x - q.p.x;
+    const float dy = p.y - q.p.y;
+    const float dz = p.z - q.p.z;
+    if (dx == 0.0f && dy == 0.0f && dz == 0.0f) {
+      rho += c.w0;  // poly6(0); f * (+-0) terms leave the sums unchanged
+      return;
+    }
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    rho += poly6(r2, c);
+    if (r2 < c.h2) {
+      const float f = pair_spiky(r2, c);
+      const float gx = f * dx, gy = f * dy, gz = f * dz;
+      gsx += gx;
+      gsy += gy;
+      gsz += gz;
+      const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
+      sg2 += jx * jx + jy * jy + jz * jz;
+    }
+  };
+  gather(
+      c, nv, cv, L, [&](int t) { return EPos{ld3(P, t)}; }, pair,
+      [&](const EPos&, int) { return true; },
+      [&](const float3& cp, bool) {
+        const float dx = p.x - cp.x;
+        const float dy = p.y - cp.y;
+        const float dz = p.z - cp.z;
+        igrad = false;
         if (dx == 0.0f && dy == 0.0f && dz == 0.0f) {
-          rho += c.w0;  // poly6(0); f * (+-0) terms leave the sums unchanged
-          return;
+          iw = c.w0;
+          return 2;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        rho += poly6(r2, c);
+        iw = poly6(r2, c);
         if (r2 < c.h2) {
           const float f = pair_spiky(r2, c);
-          const float gx = f * dx, gy = f * dy, gz = f * dz;
-          gsx += gx;
-          gsy += gy;
-          gsz += gz;
-          const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
-          sg2 += jx * jx + jy * jy + jz * jz;
+          igx = f * dx, igy = f * dy, igz = f * dz;
+          const float jx = ngs * igx, jy = ngs * igy, jz = ngs * igz;
+          ij2 = jx * jx + jy * jy + jz * jz;
+          igrad = true;
         }
+        return 2;
+      },
+      [&](const EPos&) {},
+      [&](int cnt) {
+        // cnt identical neighbors: each accumulator receives the same term
+        // cnt times, in order
+        for (int k = 0; k < cnt; ++k) rho += iw;
+        if (igrad)
+          for (int k = 0; k < cnt; ++k) {
+            gsx += igx;
+            gsy += igy;
+            gsz += igz;
+            sg2 += ij2;
+          }
       });
+  if (!L.active) return;
   rho += c.w0;
   rho *= c.mass;
   const float C = rho * c.inv_density - 1.0f;
@@ -714,43 +1109,74 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
 template <bool kSolve, bool kLast, bool kRest>
 __global__ __launch_bounds__(1024) void k_delta(
     StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext, NbrView nv,
-    const int* __restrict__ seg, const ScenePlanes* __restrict__ sp,
-    const float4* __restrict__ posB, float4* __restrict__ posA, float4* __restrict__ vout,
-    const float* __restrict__ rho, const unsigned* __restrict__ flags) {
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (s >= c.n || fatal(flags)) return;
+    CellView cv, const ScenePlanes* __restrict__ sp, const float4* __restrict__ posB,
+    float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
+    const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const LaneCell L =
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+  const int s = L.s;
   const float4 pi = P[s];
   float nx = pi.x, ny = pi.y, nz = pi.z;
-  const ScenePlanes& sc = sp[scene_of(s, seg, c.n_scenes)];
   if (kSolve) {
     float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
-    for_each_neighbor(
-        nv, s, [&](int t) { return P[t]; },
-        [&](int, const float4& q) {
-          const float dx = pi.x - q.x;
-          const float dy = pi.y - q.y;
-          const float dz = pi.z - q.z;
+    // uniform-tile geometry
+    float hdx = 0.f, hdy = 0.f, hdz = 0.f, hf = 0.f, hsc = 0.f;
+    bool hsame = false;
+    auto add_term = [&](float dx, float dy, float dz, bool same, float f, float W, float lj) {
+      float sv = pi.w + lj;
+      if (c.scorr_on) {
+        const float ratio = W * c.scorr_inv_wdq;
+        const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
+        sv += scorr;
+      }
+      const float sf = sv * f;
+      if (same && isfinite(sf)) return;  // (sv f) * (+-0): +-0
+      ddx += sf * dx;
+      ddy += sf * dy;
+      ddz += sf * dz;
+    };
+    gather(
+        c, nv, cv, L,
+        [&](int t) {
+          const float4 q = P[t];
+          return EPosL{make_float3(q.x, q.y, q.z), q.w};
+        },
+        [&](const EPosL& q) {
+          const float dx = pi.x - q.p.x;
+          const float dy = pi.y - q.p.y;
+          const float dz = pi.z - q.p.z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < c.h2) {
             const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-            const float f = same ? c.f_min : pair_spiky(r2, c);
-            float sv = pi.w + q.w;
-            if (c.scorr_on) {
-              const float W = same ? c.w0 : poly6(r2, c);
-              const float ratio = W * c.scorr_inv_wdq;
-              const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
-              sv += scorr;
-            }
-            const float sf = sv * f;
-            if (same && isfinite(sf)) return;  // (sv f) * (+-0): +-0
-            ddx += sf * dx;
-            ddy += sf * dy;
-            ddz += sf * dz;
+            add_term(dx, dy, dz, same, same ? c.f_min : pair_spiky(r2, c),
+                     same ? c.w0 : poly6(r2, c), q.l);
           }
-        });
+        },
+        // bounded lambdas keep (sv f) finite for coincident neighbors
+        [&](const EPosL& tile, int m) {
+          const int lane = threadIdx.x & (kWave - 1);
+          return tile_all(lane >= m || fabsf(tile.l) < c.lam_safe);
+        },
+        [&](const float3& cp, bool tf) {
+          hdx = pi.x - cp.x;
+          hdy = pi.y - cp.y;
+          hdz = pi.z - cp.z;
+          const float r2 = hdx * hdx + hdy * hdy + hdz * hdz;
+          if (!(r2 < c.h2)) return 1;
+          hsame = (hdx == 0.0f && hdy == 0.0f && hdz == 0.0f);
+          if (hsame && tf && fabsf(pi.w) < c.lam_safe) return 1;
+          hf = hsame ? c.f_min : pair_spiky(r2, c);
+          hsc = hsame ? c.w0 : poly6(r2, c);  // W, for s_corr
+          return 0;
+        },
+        [&](const EPosL& q) { add_term(hdx, hdy, hdz, hsame, hf, hsc, q.l); },
+        [&](int) {});
+    if (!L.active) return;
     ddx *= c.inv_density;
     ddy *= c.inv_density;
     ddz *= c.inv_density;
+    const ScenePlanes& sc = sp[scene_of(s, cv.seg, c.n_scenes)];
     if (sc.n_planes > 0) {
       float qx = pi.x + ddx, qy = pi.y + ddy, qz = pi.z + ddz;
       project_planes(qx, qy, qz, sc);
@@ -762,6 +1188,7 @@ __global__ __launch_bounds__(1024) void k_delta(
     ny = pi.y + ddy;
     nz = pi.z + ddz;
   }
+  if (!L.active) return;
   if (!kLast) {
     Pnext[s] = make_float4(nx, ny, nz, 0.0f);
     return;
@@ -771,7 +1198,7 @@ __global__ __launch_bounds__(1024) void k_delta(
   float vy = (ny - x.y) / c.dt;
   float vz = (nz - x.z) / c.dt;
   posA[s] = make_float4(nx, ny, nz, x.w);
-  if (kRest) restitute(vx, vy, vz, nx, ny, nz, sc);
+  if (kRest) restitute(vx, vy, vz, nx, ny, nz, sp[scene_of(s, cv.seg, c.n_scenes)]);
   vout[s] = make_float4(vx, vy, vz, kSolve ? rho[s] : 0.0f);
 }
 
@@ -780,34 +1207,51 @@ template <bool kRest>
 __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __restrict__ pos,
                                                const float4* __restrict__ vin,
                                                float4* __restrict__ vout, NbrView nv,
-                                               const int* __restrict__ seg,
-                                               const ScenePlanes* __restrict__ sp,
+                                               CellView cv, const ScenePlanes* __restrict__ sp,
                                                const unsigned* __restrict__ flags) {
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (s >= c.n || fatal(flags)) return;
+  if (fatal(flags)) return;
+  const LaneCell L =
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+  const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float4 vi = vin[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_each_neighbor(
-      nv, s, [&](int t) { return Pair2{pos[t], vin[t]}; },
-      [&](int, const Pair2& q) {
-        const float dx = xi.x - q.a.x;
-        const float dy = xi.y - q.a.y;
-        const float dz = xi.z - q.a.z;
+  float hW = 0.0f;
+  auto add_term = [&](float W, const EPosV& q) {
+    ax += (q.v.x - vi.x) * W * q.irho;
+    ay += (q.v.y - vi.y) * W * q.irho;
+    az += (q.v.z - vi.z) * W * q.irho;
+  };
+  gather(
+      c, nv, cv, L,
+      [&](int t) {
+        const float4 v = vin[t];
+        return EPosV{ld3(pos, t), make_float3(v.x, v.y, v.z),
+                     (v.w > 0.0f) ? (c.mass / v.w) : 0.0f};
+      },
+      [&](const EPosV& q) {
+        const float dx = xi.x - q.p.x;
+        const float dy = xi.y - q.p.y;
+        const float dz = xi.z - q.p.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < c.h2) {
-          const float4 vj = q.b;
-          const float W = poly6(r2, c);
-          const float irho = (vj.w > 0.0f) ? (c.mass / vj.w) : 0.0f;
-          ax += (vj.x - vi.x) * W * irho;
-          ay += (vj.y - vi.y) * W * irho;
-          az += (vj.z - vi.z) * W * irho;
-        }
-      });
+        if (r2 < c.h2) add_term(poly6(r2, c), q);
+      },
+      [&](const EPosV&, int) { return true; },
+      [&](const float3& cp, bool) {
+        const float dx = xi.x - cp.x;
+        const float dy = xi.y - cp.y;
+        const float dz = xi.z - cp.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < c.h2)) return 1;
+        hW = poly6(r2, c);
+        return 0;
+      },
+      [&](const EPosV& q) { add_term(hW, q); }, [&](int) {});
+  if (!L.active) return;
   float vx = vi.x + c.visc_c * ax;
   float vy = vi.y + c.visc_c * ay;
   float vz = vi.z + c.visc_c * az;
-  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, seg, c.n_scenes)]);
+  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, cv.seg, c.n_scenes)]);
   vout[s] = make_float4(vx, vy, vz, 0.0f);
 }
 
@@ -815,31 +1259,59 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
 __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4* __restrict__ pos,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ omega, NbrView nv,
+                                                     CellView cv,
                                                      const unsigned* __restrict__ flags) {
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (s >= c.n || fatal(flags)) return;
+  if (fatal(flags)) return;
+  const LaneCell L =
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+  const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float3 vi = ld3(vel, s);
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  for_each_neighbor(
-      nv, s, [&](int t) { return Pair2{pos[t], vel[t]}; },
-      [&](int, const Pair2& q) {
-        const float dx = xi.x - q.a.x;
-        const float dy = xi.y - q.a.y;
-        const float dz = xi.z - q.a.z;
+  float hgx = 0.f, hgy = 0.f, hgz = 0.f;
+  bool hsame = false;
+  auto add_term = [&](bool same, float gx, float gy, float gz, const float3& vj) {
+    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    if (same && isfinite(dvx) && isfinite(dvy) && isfinite(dvz))
+      return;  // dv x (f * (+-0)) is a signed zero
+    ox += dvy * gz - dvz * gy;
+    oy += dvz * gx - dvx * gz;
+    oz += dvx * gy - dvy * gx;
+  };
+  const bool own_ok = fabsf(vi.x) < kHalfMax && fabsf(vi.y) < kHalfMax && fabsf(vi.z) < kHalfMax;
+  gather(
+      c, nv, cv, L, [&](int t) { return EPosV3{ld3(pos, t), ld3(vel, t)}; },
+      [&](const EPosV3& q) {
+        const float dx = xi.x - q.p.x;
+        const float dy = xi.y - q.p.y;
+        const float dz = xi.z - q.p.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (r2 < c.h2) {
-          const float dvx = q.b.x - vi.x, dvy = q.b.y - vi.y, dvz = q.b.z - vi.z;
-          if (dx == 0.0f && dy == 0.0f && dz == 0.0f && isfinite(dvx) && isfinite(dvy) &&
-              isfinite(dvz))
-            return;  // dv x (f * (+-0)) is a signed zero
+          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
           const float f = pair_spiky(r2, c);
-          const float gx = f * dx, gy = f * dy, gz = f * dz;
-          ox += dvy * gz - dvz * gy;
-          oy += dvz * gx - dvx * gz;
-          oz += dvx * gy - dvy * gx;
+          add_term(same, f * dx, f * dy, f * dz, q.v);
         }
-      });
+      },
+      // finite velocity differences make coincident terms signed zeros
+      [&](const EPosV3& tile, int m) {
+        const int lane = threadIdx.x & (kWave - 1);
+        return tile_all(lane >= m || (fabsf(tile.v.x) < kHalfMax && fabsf(tile.v.y) < kHalfMax &&
+                                      fabsf(tile.v.z) < kHalfMax));
+      },
+      [&](const float3& cp, bool tf) {
+        const float dx = xi.x - cp.x;
+        const float dy = xi.y - cp.y;
+        const float dz = xi.z - cp.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < c.h2)) return 1;
+        hsame = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+        if (hsame && tf && own_ok) return 1;
+        const float f = pair_spiky(r2, c);
+        hgx = f * dx, hgy = f * dy, hgz = f * dz;
+        return 0;
+      },
+      [&](const EPosV3& q) { add_term(hsame, hgx, hgy, hgz, q.v); }, [&](int) {});
+  if (!L.active) return;
   omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
 }
 
@@ -849,35 +1321,57 @@ template <bool kRest>
 __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4* __restrict__ pos,
                                                      const float4* __restrict__ omega,
                                                      float4* __restrict__ vel, NbrView nv,
-                                                     const int* __restrict__ seg,
+                                                     CellView cv,
                                                      const ScenePlanes* __restrict__ sp,
                                                      const unsigned* __restrict__ flags) {
-  const int s = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (s >= c.n || fatal(flags)) return;
+  if (fatal(flags)) return;
+  const LaneCell L =
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+  const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float4 om = omega[s];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  for_each_neighbor(
-      nv, s,
-      [&](int t) {
-        const float3 x = ld3(pos, t);
-        return make_float4(x.x, x.y, x.z, reinterpret_cast<const float*>(omega + t)[3]);
-      },
-      [&](int, const float4& q) {
-        const float dx = xi.x - q.x;
-        const float dy = xi.y - q.y;
-        const float dz = xi.z - q.z;
+  float hgx = 0.f, hgy = 0.f, hgz = 0.f;
+  bool hsame = false;
+  auto add_term = [&](bool same, float gx, float gy, float gz, float omj) {
+    const float co = omj - om.w;
+    if (same && isfinite(co)) return;
+    ex += co * gx;
+    ey += co * gy;
+    ez += co * gz;
+  };
+  gather(
+      c, nv, cv, L,
+      [&](int t) { return EPosW{ld3(pos, t), reinterpret_cast<const float*>(omega + t)[3]}; },
+      [&](const EPosW& q) {
+        const float dx = xi.x - q.p.x;
+        const float dy = xi.y - q.p.y;
+        const float dz = xi.z - q.p.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (r2 < c.h2) {
-          const float co = q.w - om.w;
-          if (dx == 0.0f && dy == 0.0f && dz == 0.0f && isfinite(co)) return;
+          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
           const float f = pair_spiky(r2, c);
-          const float gx = f * dx, gy = f * dy, gz = f * dz;
-          ex += co * gx;
-          ey += co * gy;
-          ez += co * gz;
+          add_term(same, f * dx, f * dy, f * dz, q.w);
         }
-      });
+      },
+      [&](const EPosW& tile, int m) {
+        const int lane = threadIdx.x & (kWave - 1);
+        return tile_all(lane >= m || fabsf(tile.w) < kHalfMax);
+      },
+      [&](const float3& cp, bool tf) {
+        const float dx = xi.x - cp.x;
+        const float dy = xi.y - cp.y;
+        const float dz = xi.z - cp.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < c.h2)) return 1;
+        hsame = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+        if (hsame && tf && fabsf(om.w) < kHalfMax) return 1;
+        const float f = pair_spiky(r2, c);
+        hgx = f * dx, hgy = f * dy, hgz = f * dz;
+        return 0;
+      },
+      [&](const EPosW& q) { add_term(hsame, hgx, hgy, hgz, q.w); }, [&](int) {});
+  if (!L.active) return;
   const float len = sqrtf(ex * ex + ey * ey + ez * ez);
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
   if (len > c.vort_norm_eps) {
@@ -893,7 +1387,7 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
   float vx = v.x + c.dt * fx;
   float vy = v.y + c.dt * fy;
   float vz = v.z + c.dt * fz;
-  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, seg, c.n_scenes)]);
+  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, cv.seg, c.n_scenes)]);
   vel[s] = make_float4(vx, vy, vz, 0.0f);
 }
 
@@ -956,6 +1450,7 @@ hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel) {
 hipError_t enqueue_substep(Ctx& c) {
   if (c.n == 0) return hipSuccess;
   Launcher L{c};
+  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg};
   const StepConsts k = c.consts;
   const int bs = c.block;
   const int nb = blocks_for(c.n, bs);
@@ -994,17 +1489,17 @@ hipError_t enqueue_substep(Ctx& c) {
   }));
   FH_TRY(L("reorder", [&] {
     k_reorder<<<nb, bs, 0, st>>>(k, c.perm, c.sid, c.skey, c.start, c.seg, c.planes, c.posA,
-                                 c.velA, c.posB, c.velB, c.pred[0], c.cellc, c.diag_cell,
-                                 c.diag_key, c.flags);
+                                 c.velA, c.posB, c.velB, c.pred[0], c.cellc, c.slot_cell,
+                                 c.slot_key, c.flags);
   }));
   FH_TRY(L("neighbors", [&] {
-    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], c.seg, c.grid, c.start, c.noff, c.nwords,
+    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
   }));
 
   // velocity buffer that the commit writes: XSPH needs a second buffer to
   // stay Jacobi, so the commit goes to velB and XSPH back to velA.
-  const NbrView nv{c.noff, c.nwords, c.nbr};
+  const NbrView nv{c.noff, c.nwords, c.nbr, c.ncount};
   float4* v1 = c.xsph ? c.velB : c.velA;
   const bool rest_in_delta = c.restitution_any && !c.xsph && !c.vort;
   const int I = c.iterations;
@@ -1013,19 +1508,19 @@ hipError_t enqueue_substep(Ctx& c) {
     float4* Pn = c.pred[(it + 1) & 1];
     const bool last = (it + 1 == I);
     FH_TRY(L("lambda", [&] {
-      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, c.rho, last ? 1 : 0, c.flags);
+      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, c.flags);
     }));
     FH_TRY(L("delta", [&] {
       if (!last)
-        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
+        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
                                                        c.flags);
       else if (rest_in_delta)
-        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
+        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                      c.planes, c.posB, c.posA, v1, c.rho,
                                                      c.flags);
       else
-        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, c.seg,
+        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
                                                       c.flags);
     }));
@@ -1033,36 +1528,37 @@ hipError_t enqueue_substep(Ctx& c) {
   if (I == 0) {
     FH_TRY(L("delta", [&] {
       if (rest_in_delta)
-        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv,
-                                                      c.seg, c.planes, c.posB, c.posA, v1,
-                                                      c.rho, c.flags);
+        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
+                                                      c.planes, c.posB, c.posA, v1, c.rho,
+                                                      c.flags);
       else
-        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, c.seg, c.planes, c.posB, c.posA,
-                                                       v1, c.rho, c.flags);
+        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
+                                                       c.planes, c.posB, c.posA, v1, c.rho,
+                                                       c.flags);
     }));
   }
   if (c.xsph) {
     const bool rest = c.restitution_any && !c.vort;
     FH_TRY(L("xsph", [&] {
       if (rest)
-        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, c.seg,
+        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                         c.planes, c.flags);
       else
-        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, c.seg,
+        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                          c.planes, c.flags);
     }));
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
-      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, c.flags);
+      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
     }));
     FH_TRY(L("vorticity_apply", [&] {
       if (c.restitution_any)
-        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv,
-                                              c.seg, c.planes, c.flags);
+        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
+                                              c.planes, c.flags);
       else
-        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv,
-                                               c.seg, c.planes, c.flags);
+        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
+                                               c.planes, c.flags);
     }));
   }
 #undef FH_TRY

@@ -42,6 +42,7 @@ class FluidHipConfig(C.Structure):
     _fields_ = [
         ("cell_capacity", C.c_int64), ("neighbor_capacity", C.c_int64), ("device", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("record_diag", C.c_int32),
+        ("wave_mode", C.c_int32),
     ]
 
 
@@ -156,7 +157,7 @@ class HipSolver:
     counts: particle count per scene (defaults to [state.n] via upload)."""
 
     def __init__(self, params, counts, *, cell_capacity=0, neighbor_capacity=0, block_size=256,
-                 xcd_remap=True, record_diag=False, device=0):
+                 xcd_remap=True, record_diag=False, device=0, wave_mode=0):
         self.lib = load_library()
         plist = list(params) if isinstance(params, (list, tuple)) else [params]
         counts = [int(c) for c in counts]
@@ -171,7 +172,8 @@ class HipSolver:
         arr = (FluidHipParams * self.n_scenes)(*[pack_params(p) for p in plist])
         cnt = (C.c_int64 * self.n_scenes)(*counts)
         cfg = FluidHipConfig(int(cell_capacity), int(neighbor_capacity), int(device),
-                             int(block_size), int(bool(xcd_remap)), int(bool(record_diag)))
+                             int(block_size), int(bool(xcd_remap)), int(bool(record_diag)),
+                             int(wave_mode))
         h = C.c_void_p()
         _check(self.lib.fluid_hip_create(C.byref(h), self.n_scenes, cnt, arr, C.byref(cfg)))
         self.h = h

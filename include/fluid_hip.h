@@ -81,6 +81,9 @@ typedef struct fluid_hip_config {
   int32_t block_size;         /* threads per workgroup of the particle kernels (64..1024) */
   int32_t xcd_remap;          /* 1: map consecutive particle tiles to one XCD */
   int32_t record_diag;        /* 1: keep per-step grid diagnostics (slower) */
+  int32_t wave_mode;          /* gathers of wavefronts that share one cell: 0 default (2),
+                                 1 per-lane lists only, 2 shared candidate tiles,
+                                 3 shared tiles without the coincident-tile shortcut */
 } fluid_hip_config;
 
 typedef struct fluid_hip_ctx fluid_hip_ctx;

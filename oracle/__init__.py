@@ -97,14 +97,16 @@ def _pack_params(p, mutation=0):
 
 
 def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
-         dtype=np.float32, mutation=0, threads=None, params64=None):
+         dtype=np.float32, mutation=0, threads=None, params64=None, nbr_cap=None):
     """Run `nsteps` CPU steps.  pos, vel: (3, n) arrays.  Returns a dict with
     pos, vel, time and, with diag=True, the last step's grid/neighbor/rho data
     (original particle order).
 
     which: "oracle" (this project's restatement) or "ref" (the reference
     core.cpp).  dtype float64 runs the FP64 oracle; `params64` then supplies
-    double-precision overrides (e.g. gravity for finite differences)."""
+    double-precision overrides (e.g. gravity for finite differences).
+    nbr_cap sizes the neighbor-list buffer of the diagnostics (default 64
+    per particle; grown automatically when the lists are longer)."""
     n = int(pos.shape[1])
     pos = np.ascontiguousarray(pos, dtype=dtype).copy()
     vel = np.ascontiguousarray(vel, dtype=dtype).copy()
@@ -119,7 +121,7 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
     out = {}
     d = None
     if diag:
-        cap = max(1, n * 64)
+        cap = int(nbr_cap) if nbr_cap else max(1, n * 64)
         out["cell"] = np.zeros(3 * n, np.int32)
         out["sorted_ids"] = np.zeros(n, np.int32)
         out["nbr_count"] = np.zeros(n, np.int32)
@@ -147,11 +149,17 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
     else:
         raise ValueError(which)
     fn.restype = C.c_int
+    pos_in, vel_in = pos.copy(), vel.copy()
     rc = fn(C.byref(op), C.c_longlong(n), pos.ctypes.data_as(C.c_void_p),
             vel.ctypes.data_as(C.c_void_p), C.c_int(nsteps), t.ctypes.data_as(C.c_void_p),
             C.byref(d) if d is not None else None)
     if rc != 0:
         raise RuntimeError(f"{which} step failed rc={rc}")
+    if d is not None and d.nbr_total > d.nbr_cap:
+        # dense scenes: rerun with a list buffer of the exact size
+        return step(params, pos_in, vel_in, nsteps=nsteps, time=time, diag=True, which=which,
+                    dtype=dtype, mutation=mutation, threads=threads, params64=params64,
+                    nbr_cap=int(d.nbr_total))
     out["pos"], out["vel"], out["time"] = pos, vel, t[0]
     if diag:
         out["cell"] = out["cell"].reshape(n, 3)

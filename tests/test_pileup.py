@@ -1,0 +1,102 @@
+"""Dense-cell regime: with these parameters the dam break piles particles
+onto walls and corners (hundreds to tens of thousands per cell, mostly
+exactly coincident).  Wavefronts that share a cell take the shared-tile and
+coincident-tile paths there; every wave mode must stay bitwise equal to the
+CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from fluidsimulator_amd import hip, scenes
+from fluidsimulator_amd.params import Params
+from tests.helpers import same_bits
+
+pytestmark = pytest.mark.gpu
+
+WAVE_MODES = {1: "per-lane lists", 2: "shared tiles + coincident tiles", 3: "shared tiles"}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    hip.load_library()
+    ok, _, err = hip.hip_device_available()
+    assert ok, f"no HIP device: {err}"
+    oracle.build(ref=False)
+
+
+@pytest.fixture(scope="module")
+def pileup_reference():
+    # 131,072 particles, all flags; by step 40 cells hold several hundred
+    # coincident particles.
+    sc = scenes.million_block(dims=(64, 64, 32))
+    checkpoints = (10, 25, 40)
+    out, pos, vel, prev = {}, sc.state.pos, sc.state.vel, 0
+    for k in checkpoints:
+        r = oracle.step(sc.params, pos, vel, nsteps=k - prev)
+        pos, vel, prev = r["pos"], r["vel"], k
+        out[k] = (pos, vel)
+    return sc, out
+
+
+@pytest.mark.parametrize("mode", sorted(WAVE_MODES))
+def test_pileup_rollout_bitwise(pileup_reference, mode):
+    sc, ref = pileup_reference
+    with hip.HipSolver(sc.params, [sc.state.n], wave_mode=mode) as s:
+        s.upload(sc.state.pos, sc.state.vel)
+        prev = 0
+        for k, (pos, vel) in ref.items():
+            s.step(k - prev)
+            prev = k
+            st = s.download()
+            assert same_bits(st.pos, pos), (WAVE_MODES[mode], k)
+            assert same_bits(st.vel, vel), (WAVE_MODES[mode], k)
+
+
+def _clusters(seed=3):
+    # coincident clusters in neighboring cells (some cells hold two
+    # clusters), with scattered particles around them
+    rng = np.random.default_rng(seed)
+    pts = []
+    centers = [(0.52, 0.51, 0.53), (0.58, 0.51, 0.53), (0.45, 0.43, 0.61), (0.55, 0.66, 0.47),
+               (0.52, 0.51, 0.44), (0.61, 0.59, 0.52)]
+    for k, c in enumerate(centers):
+        pts.append(np.repeat(np.array(c, np.float32)[:, None], 90 + 20 * k, axis=1))
+    pts.append((0.53 + 0.06 * rng.standard_normal((3, 300))).astype(np.float32))
+    pos = np.concatenate(pts, axis=1)
+    perm = rng.permutation(pos.shape[1])  # interleave ids across clusters
+    return np.ascontiguousarray(pos[:, perm])
+
+
+@pytest.mark.parametrize("mode", sorted(WAVE_MODES))
+@pytest.mark.parametrize("flags", [False, True])
+def test_coincident_clusters_bitwise(mode, flags):
+    p = scenes.base_params((1.0, 1.0, 1.0), flags)
+    pos = _clusters()
+    vel = np.zeros_like(pos)
+    vel[1] = -0.5
+    o = oracle.step(p, pos, vel, nsteps=3, diag=True)
+    with hip.HipSolver(p, [pos.shape[1]], wave_mode=mode, record_diag=True) as s:
+        s.upload(pos, vel)
+        s.step(3)
+        st = s.download()
+        d = s.diag()
+        nb = s.neighbors()
+    assert np.array_equal(d["nbr_count"], o["nbr_count"])
+    assert np.array_equal(nb, o["nbr_ids"])
+    assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"])
+
+
+def test_all_particles_at_one_point():
+    p = Params(dt=float(np.float32(1 / 120)), density=1000.0, particle_mass=0.125, h=0.1,
+               epsilon=100.0, solver_iterations=4, external_forces=(0.0, -9.8, 0.0),
+               enable_xsph=True, visc_c=0.01, enable_vorticity=True, vort_epsilon=5e-4)
+    p.set_box(1.0, 1.0, 1.0)
+    pos = np.full((3, 700), 0.5, np.float32)
+    vel = np.zeros_like(pos)
+    o = oracle.step(p, pos, vel, nsteps=2)
+    for mode in sorted(WAVE_MODES):
+        with hip.HipSolver(p, [pos.shape[1]], wave_mode=mode) as s:
+            s.upload(pos, vel)
+            s.step(2)
+            st = s.download()
+        assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"]), WAVE_MODES[mode]
