@@ -51,7 +51,20 @@ def sync_all():
         torch.cuda.synchronize()
 
 
-def cpu_baseline(state, params, threads, steps=2, warm=1):
+def measured_traffic(kernel: str, dims) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter's reading for a device copy of known size), if present."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_million_block.json")
+    if tuple(dims) != (128, 128, 64) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    k = data.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["hbm_bytes_per_launch"])
+
+
+def cpu_baseline(state, params, threads, steps=1, warm=0):
     """The reference's own CPU step (oracle/_ref, built from core.cpp) on the
     state the GPU timed region starts from."""
     import oracle
@@ -74,6 +87,57 @@ def cpu_baseline(state, params, threads, steps=2, warm=1):
     }
 
 
+def copy_calibration(mbytes: int) -> None:
+    """One device-to-device copy of `mbytes` MiB (the runtime's blit kernel):
+    the reference point that scales FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip_rt = C.CDLL("libamdhip64.so")
+    n = int(mbytes) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip_rt.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip_rt.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip_rt.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip_rt.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # device to device
+    assert hip_rt.hipDeviceSynchronize() == 0
+    hip_rt.hipFree(a)
+    hip_rt.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}))
+
+
+def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
+    """Text summary of a `rocprofv3 --kernel-trace --stats` run of this
+    script (rocpd database output): per-kernel totals over every dispatch of
+    the command, and per-kernel averages over the last `steps` substeps
+    (the timed window) from the dispatch trace."""
+    import glob
+    import sqlite3
+
+    dbs = sorted(glob.glob(os.path.join(outdir, "**", "*_results.db"), recursive=True))
+    if not dbs:
+        return f"no rocpd database under {outdir}"
+    cur = sqlite3.connect(dbs[0]).cursor()
+    rows = cur.execute("select name, count(*), avg(duration), sum(duration) from kernels "
+                       "group by name order by sum(duration) desc").fetchall()
+    total = sum(r[3] for r in rows) or 1
+    lines = ["all dispatches of the command (rocprofv3 --stats view):",
+             f"{'kernel':28s} {'calls':>7s} {'avg_us':>12s} {'total_ms':>12s} {'pct':>6s}"]
+    for name, n, avg, tot in rows:
+        short = name.split("::")[-1].split("(")[0].split("<")[0][:28]
+        lines.append(f"{short:28s} {n:7d} {avg / 1e3:12.1f} {tot / 1e6:12.1f} "
+                     f"{100.0 * tot / total:6.2f}")
+    lines += ["", f"timed window: last {steps} substeps of each kernel's dispatches:"]
+    for key, per_step in launches_per_step.items():
+        d = [r[0] for r in cur.execute(
+            "select duration from kernels where name like ? order by start", (f"%k_{key}%",))]
+        take = steps * per_step
+        if d and take:
+            w = d[-take:]
+            lines.append(f"  {key:18s} launches {len(w):6d}  avg_us {sum(w) / len(w) / 1e3:12.1f}")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,7 +149,19 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--dims", type=str, default="128,128,64")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run a device copy of this size (counter calibration)")
+    ap.add_argument("--summarize-rocprof", type=str, default="", metavar="DIR",
+                    help="print a summary of a rocprofv3 output directory of this script")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.summarize_rocprof:
+        print(summarize_rocprof(args.summarize_rocprof, args.steps,
+                                {"lambda": 4, "delta": 4, "xsph": 1, "vort_omega": 1,
+                                 "vort_apply": 1, "neighbors": 1, "reorder": 1}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "layout is a design note, not a measured path", file=sys.stderr)
@@ -105,10 +181,15 @@ def main():
         s.profile(True)
     sync_all()
     t0 = time.perf_counter()
+    grows_before = s.stats()["grow_events"]
     s.step(args.steps, graph=args.graph)  # one sync at the end; grows tables if needed
     sync_all()
     elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed
+    st = s.stats()
+    sizes = {"neighbor_pairs_last_step": st["pairs"], "list_words_last_step": st["words"],
+             "grid_cells_last_step": st["cells"],
+             "capacity_growths_in_window": st["grow_events"] - grows_before}
     kernels = {}
     if prof:
         for k in KERNELS:
@@ -126,7 +207,8 @@ def main():
         avg_s = kernels[dom]["avg_us"] * 1e-6
         achieved = KERNEL_BYTES[dom] * n / avg_s / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(dom, dims),
                     "bytes_per_launch": KERNEL_BYTES[dom] * n}
         for k, v in kernels.items():
             if k in KERNEL_BYTES:
@@ -159,6 +241,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "sizes": sizes,
     }
     print(json.dumps(out))
 

@@ -444,10 +444,14 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.steps_started, 1);
   ALLOC(c.stage, 6 * n);
 #undef ALLOC
+  // Defaults leave room for the spray and pile-up of violent scenes (cell
+  // tables of tens of millions of cells, run-length lists well above the
+  // particle count); both grow on demand.
   const long long cell_cap =
-      cfg.cell_capacity > 0 ? cfg.cell_capacity : std::max<long long>(1ll << 22, 8ll * c.n);
+      cfg.cell_capacity > 0 ? cfg.cell_capacity
+                            : std::min<long long>(INT_MAX - 1, std::max<long long>(1ll << 24, 24ll * c.n));
   const long long nbr_cap =
-      cfg.neighbor_capacity > 0 ? cfg.neighbor_capacity : std::max<long long>(1ll << 16, 40ll * c.n);
+      cfg.neighbor_capacity > 0 ? cfg.neighbor_capacity : std::max<long long>(1ll << 20, 64ll * c.n);
   if (cell_cap >= INT_MAX) return bail(fail(FLUID_HIP_EINVAL, "cell capacity >= 2^31"));
   if ((rc = alloc_cells(c, cell_cap))) return bail(rc);
   if ((rc = alloc_neighbors(c, nbr_cap))) return bail(rc);
@@ -603,18 +607,20 @@ int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
     // did not complete (the state of a failed step is left untouched).
     if (st.flags & fluidhip::kFlagCells) {
       const long long need = static_cast<long long>(st.need_cells);
-      const long long cap = need + need / 4 + 4096;
-      if (cap >= INT_MAX) return fail(FLUID_HIP_ECAPACITY, "cell table would exceed 2^31 cells");
-      if ((rc = alloc_cells(c, cap))) return rc;
+      const long long cap = std::max(need + need / 2 + 4096, 2 * c.cell_cap);
+      if (need + 4096 >= INT_MAX)
+        return fail(FLUID_HIP_ECAPACITY, "cell table would exceed 2^31 cells");
+      if ((rc = alloc_cells(c, std::min<long long>(cap, INT_MAX - 1)))) return rc;
     } else {
       const long long need = static_cast<long long>(st.nbr_total);
-      const long long cap = need + need / 4 + 4096;
+      const long long cap = std::max(need + need / 2 + 4096, 2 * c.nbr_cap);
       if ((rc = alloc_neighbors(c, cap))) {
         g_last_error = "neighbor list of " + std::to_string(cap) + " words: " + g_last_error;
         return FLUID_HIP_ECAPACITY;
       }
     }
     drop_graph(c);
+    ++c.grow_events;
   }
   if (remaining > 0) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
   return FLUID_HIP_OK;
@@ -738,7 +744,7 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
   if (rc) return rc;
   Ctx& c = ctx->c;
   if (!out || n < 0) return fail(FLUID_HIP_EINVAL, "bad output");
-  int64_t v[5] = {0, 0, 0, c.nbr_cap, c.cell_cap};
+  int64_t v[6] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events};
   if (c.stepped && c.n > 0) {
     std::vector<int> cnt(c.n);
     HIP_TRY(hipMemcpy(cnt.data(), c.ncount, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read counts");
@@ -750,7 +756,7 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
     HIP_TRY(hipMemcpy(&cells, c.total_cells, sizeof(int), hipMemcpyDeviceToHost), "read cells");
     v[2] = cells;
   }
-  for (int i = 0; i < n && i < 5; ++i) out[i] = v[i];
+  for (int i = 0; i < n && i < 6; ++i) out[i] = v[i];
   return FLUID_HIP_OK;
 }
 

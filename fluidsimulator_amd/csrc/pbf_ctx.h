@@ -91,6 +91,7 @@ struct Ctx {
   float time = 0.0f;
   bool uploaded = false;
   bool stepped = false;
+  long long grow_events = 0;  // capacity growths (each replays the failed substep)
 
   // staging for transfers
   float* stage = nullptr;  // 6 * n

@@ -368,6 +368,32 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
   }
 }
 
+// Lanes whose own cell holds at least this many particles are served by
+// wavefront-cooperative walks (one per distinct dense cell in the
+// wavefront); the others walk alone.
+constexpr int kDenseCell = 32;
+// ... and so are lanes with at least this many neighbors (or, while the
+// lists are built, candidates).
+constexpr int kHeavyList = 256;
+
+// Calls fn(leader, mine) once per distinct key among the lanes with
+// want == true: `leader` is the lowest such lane holding the key and `mine`
+// marks the lanes that share it.  Wavefront-uniform control flow; every
+// lane must be present.
+template <typename Fn>
+__device__ __forceinline__ void for_each_key_group(bool want, int key, Fn&& fn) {
+  bool pend = want;
+  while (true) {
+    const unsigned long long m = __ballot(pend);
+    if (m == 0ull) break;
+    const int leader = __builtin_ctzll(m);
+    const int kl = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = pend && key == kl;
+    fn(leader, mine);
+    pend = pend && !mine;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
                                                      const uint64_t* __restrict__ cellc,
                                                      const int* __restrict__ seg,
@@ -375,13 +401,29 @@ __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
                                                      int* __restrict__ count, int* __restrict__ key,
                                                      int* __restrict__ rank,
                                                      const unsigned* __restrict__ flags) {
-  const int i = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (i >= c.n || fatal(flags)) return;
+  const int i0 = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (fatal(flags)) return;
+  const bool active = i0 < c.n;
+  const int i = active ? i0 : c.n - 1;
   const GridDesc g = grid[scene_of(i, seg, c.n_scenes)];
   int x, y, z;
   unpack_cell(cellc[i], x, y, z);
   const int k = cell_key(x - g.ox, y - g.oy, z - g.oz, g);
-  rank[i] = atomicAdd(count + k, 1);
+  // Counting sort, wavefront-aggregated: the particles arrive in the
+  // previous substep's sorted order, so neighboring lanes mostly share a
+  // cell.  Each group of lanes with one key takes one atomic for all of
+  // them and ranks itself by a ballot prefix.
+  const int lane = threadIdx.x & (kWave - 1);
+  int r = 0;
+  for_each_key_group(active, k, [&](int leader, bool mine) {
+    const unsigned long long gm = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count + k, __popcll(gm));
+    base = __builtin_amdgcn_readlane(base, leader);
+    r = mine ? base + __popcll(gm & ((1ull << lane) - 1ull)) : r;
+  });
+  if (!active) return;
+  rank[i] = r;
   key[i] = k;
 }
 
@@ -459,32 +501,6 @@ __global__ __launch_bounds__(1024) void k_scatter(StepConsts c, const int* __res
   perm[slot] = p;
   sid[slot] = __float_as_int(reinterpret_cast<const float*>(pos + p)[3]);
   skey[slot] = k;
-}
-
-// Lanes whose own cell holds at least this many particles are served by
-// wavefront-cooperative walks (one per distinct dense cell in the
-// wavefront); the others walk alone.
-constexpr int kDenseCell = 32;
-// ... and so are lanes with at least this many neighbors (or, while the
-// lists are built, candidates).
-constexpr int kHeavyList = 256;
-
-// Calls fn(leader, mine) once per distinct key among the lanes with
-// want == true: `leader` is the lowest such lane holding the key and `mine`
-// marks the lanes that share it.  Wavefront-uniform control flow; every
-// lane must be present.
-template <typename Fn>
-__device__ __forceinline__ void for_each_key_group(bool want, int key, Fn&& fn) {
-  bool pend = want;
-  while (true) {
-    const unsigned long long m = __ballot(pend);
-    if (m == 0ull) break;
-    const int leader = __builtin_ctzll(m);
-    const int kl = __builtin_amdgcn_readlane(key, leader);
-    const bool mine = pend && key == kl;
-    fn(leader, mine);
-    pend = pend && !mine;
-  }
 }
 
 // Rank by original id inside the cell, then gather the particle into its
@@ -1217,6 +1233,7 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
   const float4 vi = vin[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float hW = 0.0f;
+  float3 tv = make_float3(0.f, 0.f, 0.f);
   auto add_term = [&](float W, const EPosV& q) {
     ax += (q.v.x - vi.x) * W * q.irho;
     ay += (q.v.y - vi.y) * W * q.irho;
@@ -1236,14 +1253,27 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (r2 < c.h2) add_term(poly6(r2, c), q);
       },
-      [&](const EPosV&, int) { return true; },
-      [&](const float3& cp, bool) {
+      // one velocity in the whole tile (bitwise), finite m/rho everywhere
+      [&](const EPosV& tile, int m) {
+        const int lane = threadIdx.x & (kWave - 1);
+        tv = make_float3(bcast(tile.v.x, 0), bcast(tile.v.y, 0), bcast(tile.v.z, 0));
+        return tile_all(lane >= m || (__float_as_int(tile.v.x) == __float_as_int(tv.x) &&
+                                      __float_as_int(tile.v.y) == __float_as_int(tv.y) &&
+                                      __float_as_int(tile.v.z) == __float_as_int(tv.z) &&
+                                      isfinite(tile.irho)));
+      },
+      [&](const float3& cp, bool tf) {
         const float dx = xi.x - cp.x;
         const float dy = xi.y - cp.y;
         const float dz = xi.z - cp.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (!(r2 < c.h2)) return 1;
         hW = poly6(r2, c);
+        // (vj - vi) = +0 exactly for every neighbor: each term is a zero and
+        // leaves the sums unchanged (finite W and m/rho)
+        if (tf && vi.x == tv.x && vi.y == tv.y && vi.z == tv.z && isfinite(vi.x) &&
+            isfinite(vi.y) && isfinite(vi.z) && isfinite(hW))
+          return 1;
         return 0;
       },
       [&](const EPosV& q) { add_term(hW, q); }, [&](int) {});

@@ -263,10 +263,10 @@ class HipSolver:
 
     def stats(self):
         """Sizes of the last step: neighbor pairs, encoded list words, grid
-        cells, list capacity (words), cell capacity."""
-        out = (C.c_int64 * 5)()
-        _check(self.lib.fluid_hip_get_stats(self.h, out, 5))
-        keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity")
+        cells, list capacity (words), cell capacity, capacity growths."""
+        out = (C.c_int64 * 6)()
+        _check(self.lib.fluid_hip_get_stats(self.h, out, 6))
+        keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events")
         return dict(zip(keys, [int(x) for x in out]))
 
     def profile(self, enable=True):

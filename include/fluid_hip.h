@@ -170,7 +170,8 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
 
 /* Sizes of the last step: out[0] neighbor pairs (decoded), out[1] encoded
  * list words, out[2] grid cells (all scenes), out[3] neighbor-list capacity
- * (words), out[4] cell-table capacity.  `n` entries are written (<= 5). */
+ * (words), out[4] cell-table capacity, out[5] capacity growths since the
+ * context was created.  `n` entries are written (<= 6). */
 int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
 
 /* ---- timing -------------------------------------------------------------- */
