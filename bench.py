@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -124,7 +125,8 @@ def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
     lines = ["all dispatches of the command (rocprofv3 --stats view):",
              f"{'kernel':28s} {'calls':>7s} {'avg_us':>12s} {'total_ms':>12s} {'pct':>6s}"]
     for name, n, avg, tot in rows:
-        short = name.split("::")[-1].split("(")[0].split("<")[0][:28]
+        m = re.search(r"\b(k_[A-Za-z0-9_]+)(<[^>(]*>)?", name)
+        short = (m.group(1) + (m.group(2) or "") if m else name.split("(")[0])[:28]
         lines.append(f"{short:28s} {n:7d} {avg / 1e3:12.1f} {tot / 1e6:12.1f} "
                      f"{100.0 * tot / total:6.2f}")
     lines += ["", f"timed window: last {steps} substeps of each kernel's dispatches:"]
@@ -138,22 +140,215 @@ def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
     return "\n".join(lines)
 
 
+def bench_ensemble(args) -> dict:
+    """Segmented batch against a per-scene loop: `args.scenes` independent dam
+    breaks of uneven size (lattice dims uniform in 8..32, |g| in [8,12],
+    per-scene v0 in [-1,1]^3, all flags) stepped `args.steps` substeps
+    (a) as one batched context replaying a hipGraph per substep,
+    (b) as the same context without the graph,
+    (c) one context per scene, each stepped through all substeps in turn, once
+        eagerly and once replaying its own graph (the speed-up is quoted
+        against the faster of the two).
+    Final states of (a), (b) and (c) must be bitwise equal."""
+    ens = scenes.ensemble(n_scenes=args.scenes)
+    counts = [st.n for st in ens.states]
+    n = int(sum(counts))
+    pos = np.concatenate([st.pos for st in ens.states], axis=1)
+    vel = np.concatenate([st.vel for st in ens.states], axis=1)
+
+    def run_batched(graph):
+        s = hip.HipSolver(ens.params, counts, block_size=args.block,
+                          xcd_remap=bool(args.xcd_remap))
+        s.upload(pos, vel, 0.0)
+        s.step(args.warmup, graph=graph)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.steps, graph=graph)
+        s.sync()
+        dt = time.perf_counter() - t0
+        out = s.download()
+        s.close()
+        return dt, out
+
+    t_graph, out_graph = run_batched(True)
+    t_eager, out_eager = run_batched(False)
+
+    # one context per scene (created before the timed region, like the batch),
+    # looped eagerly and then with a captured graph per scene
+    solos = [hip.HipSolver(p, [st.n], cell_capacity=1 << 17,
+                           neighbor_capacity=64 * st.n + 4096, block_size=args.block,
+                           xcd_remap=bool(args.xcd_remap))
+             for p, st in zip(ens.params, ens.states)]
+    off = np.concatenate([[0], np.cumsum(counts)])
+    same = True
+    t_loops = {}
+    for graph in (False, True):
+        for s, st in zip(solos, ens.states):
+            s.upload(st.pos, st.vel, 0.0)
+            s.step(args.warmup, graph=graph)
+            s.sync()
+        t0 = time.perf_counter()
+        for s in solos:
+            s.step(args.steps, graph=graph)
+        t_loops[graph] = time.perf_counter() - t0
+        for i, s in enumerate(solos):
+            o = s.download()
+            sl = slice(off[i], off[i + 1])
+            same &= bool(np.array_equal(o.pos.view(np.uint32),
+                                        out_graph.pos[:, sl].view(np.uint32)))
+            same &= bool(np.array_equal(o.vel.view(np.uint32),
+                                        out_graph.vel[:, sl].view(np.uint32)))
+    for s in solos:
+        s.close()
+    t_loop = min(t_loops.values())
+    same &= bool(np.array_equal(out_eager.pos.view(np.uint32), out_graph.pos.view(np.uint32)))
+    same &= bool(np.array_equal(out_eager.vel.view(np.uint32), out_graph.vel.view(np.uint32)))
+
+    sps = args.steps / t_graph
+    return {
+        "metric": "ensemble substeps/s (all scenes per substep)",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * t_graph / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattices)",
+        "config": {"workload": f"ensemble {args.scenes} scenes, dims U{{8..32}}^3, all flags, "
+                               "4 iters, box 3^3", "particles": n, "scenes": args.scenes},
+        "particle_steps_per_s": sps * n,
+        "batched_graph_ms_per_step": 1000.0 * t_graph / args.steps,
+        "batched_eager_ms_per_step": 1000.0 * t_eager / args.steps,
+        "per_scene_loop_ms_per_step": 1000.0 * t_loops[False] / args.steps,
+        "per_scene_graph_loop_ms_per_step": 1000.0 * t_loops[True] / args.steps,
+        "speedup_vs_per_scene_loop": t_loop / t_graph,
+        "graph_vs_eager": t_eager / t_graph,
+        "bitwise_equal_to_per_scene": same,
+    }
+
+
+def adjoint_run(args, steps, coefficients) -> dict:
+    """Differentiable rollout: the 65,536-particle block (64x32x32, d=0.05,
+    +-0.1d jitter) in [0,4]x[0,3]x[0,2], s_corr + XSPH, no vorticity,
+    per-particle v0 ~ N(0, 0.5^2), `steps` substeps.  Loss = mean squared
+    distance of the final positions to a target cloud (a forward run from a
+    second seeded velocity field).  Times a plain forward rollout, the
+    recorded forward (checkpoint every `args.every`) and the reverse pass
+    (which recomputes all but the last segment)."""
+    sc = scenes.rollout_scene(steps=steps, coefficients=coefficients)
+    tg = scenes.rollout_scene(seed=6, steps=steps, coefficients=coefficients)
+    p, n = sc.params, sc.state.n
+    s = hip.HipSolver(p, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap))
+    s.upload(tg.state.pos, tg.state.vel)
+    s.step(steps)
+    target = s.download().pos.astype(np.float64)
+
+    def timed(fn):
+        s.sync()
+        t0 = time.perf_counter()
+        r = fn()
+        s.sync()
+        return time.perf_counter() - t0, r
+
+    def forward():
+        s.upload(sc.state.pos, sc.state.vel)
+        return timed(lambda: s.step(steps))[0]
+
+    forward()  # warm-up (capacity growth, code load)
+    t_fwd = min(forward() for _ in range(2))
+    s.upload(sc.state.pos, sc.state.vel)
+    s.forward_record(steps, args.every)  # warm-up of the recording buffers
+    s.backward(np.zeros((3, n), np.float32))
+    s.upload(sc.state.pos, sc.state.vel)
+    t_rec, _ = timed(lambda: s.forward_record(steps, args.every))
+    fin = s.download()
+    diff = fin.pos.astype(np.float64) - target
+    loss = float(np.mean(np.sum(diff * diff, axis=0)))
+    seed = ((2.0 / n) * diff).astype(np.float32)
+    t_bwd, (dx0, dv0, dg) = timed(lambda: s.backward(seed, None))
+    st = s.stats()
+    s.close()
+    finite = float(np.mean(np.isfinite(dv0)))
+    return {
+        "coefficients": coefficients,
+        "particles": n,
+        "ratio": (t_rec + t_bwd) / t_fwd,
+        "forward_s": t_fwd,
+        "recorded_forward_s": t_rec,
+        "backward_s": t_bwd,
+        "backward_vs_forward": t_bwd / t_fwd,
+        "recomputed_substeps": st["recomputed"],
+        "record_bytes": st["record_bytes"],
+        "loss": loss,
+        "grad_finite_fraction": finite,
+        "grad_v0_max_abs": float(np.max(np.abs(dv0[np.isfinite(dv0)]))) if finite > 0 else None,
+        "grad_gravity": dg[0].tolist(),
+        "pairs_last_step": st["pairs"],
+    }
+
+
+def bench_adjoint(args) -> dict:
+    """Adjoint against forward time on the differentiable-rollout workload
+    with the reference's default relaxation and s_corr/XSPH coefficients
+    (the gradient of 100 substeps is finite there), and the same timing with
+    the dam-break coefficients, whose 100-substep gradient overflows float32
+    (reported, not the headline)."""
+    steps = args.steps
+    main_run = adjoint_run(args, steps, "reference")
+    stated = adjoint_run(args, steps, "stated")
+    return {
+        "metric": "adjoint/forward time ratio (gradient of a 100-step rollout vs the rollout)",
+        "value": main_run["ratio"],
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": 1000.0 * (main_run["recorded_forward_s"] + main_run["backward_s"]) / steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice, seeded N(0, 0.5^2) velocities)",
+        "config": {"workload": "rollout 64x32x32 s_corr+XSPH, 4 iters, box 4x3x2, "
+                               "eps 600, k 5e-5, c 2e-4",
+                   "particles": main_run["particles"], "checkpoint_every": args.every},
+        "run": main_run,
+        "stated_coefficients_run": stated,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=120)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed substeps (default: 120 million_block, 240 ensemble, 100 adjoint)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed substeps first (default: 30 million_block, 0 ensemble)")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--dims", type=str, default="128,128,64")
+    ap.add_argument("--workload", choices=["million_block", "ensemble", "adjoint"],
+                    default="million_block",
+                    help="ensemble: batched + hipGraph against a per-scene loop; adjoint: "
+                         "gradient of a rollout against the rollout (each its own JSON line)")
+    ap.add_argument("--every", type=int, default=10, help="adjoint checkpoint interval")
+    ap.add_argument("--scenes", type=int, default=256, help="ensemble size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (counter calibration)")
     ap.add_argument("--summarize-rocprof", type=str, default="", metavar="DIR",
                     help="print a summary of a rocprofv3 output directory of this script")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"million_block": 120, "ensemble": 240, "adjoint": 100}[args.workload]
+    if args.warmup is None:
+        args.warmup = 30 if args.workload == "million_block" else 0
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -166,8 +361,14 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "layout is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args)))
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(bench_adjoint(args)))
+        return
 
-    dims = tuple(int(x) for x in args.dims.split(","))
+    dims =tuple(int(x) for x in args.dims.split(","))
     sc = scenes.million_block(dims=dims)
     n = sc.state.n
     s = hip.HipSolver(sc.params, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap))

@@ -208,8 +208,11 @@ void drop_graph(Ctx& c) {
   c.graph = nullptr;
 }
 
+void free_adj(Ctx& c);
+
 void free_all(Ctx& c) {
   drop_graph(c);
+  free_adj(c);
   for (float4** p : {&c.posA, &c.velA, &c.posB, &c.velB, &c.pred[0], &c.pred[1], &c.omega})
     dfree(*p);
   dfree(c.rho);
@@ -327,6 +330,162 @@ int check_ctx(fluid_hip_ctx* ctx) {
   if (!ctx) return fail(FLUID_HIP_EINVAL, "context is null");
   HIP_TRY(hipSetDevice(ctx->c.device), "hipSetDevice");
   return FLUID_HIP_OK;
+}
+
+// After a substep that raised a capacity flag: grow the table it reported
+// (geometrically, at least to 1.5x the need) so that the substep can be
+// replayed; the committed state of a failed substep is left untouched.
+int grow_tables(Ctx& c, const DevStatus& st) {
+  int rc;
+  if (st.flags & fluidhip::kFlagCells) {
+    const long long need = static_cast<long long>(st.need_cells);
+    const long long cap = std::max(need + need / 2 + 4096, 2 * c.cell_cap);
+    if (need + 4096 >= INT_MAX)
+      return fail(FLUID_HIP_ECAPACITY, "cell table would exceed 2^31 cells");
+    if ((rc = alloc_cells(c, std::min<long long>(cap, INT_MAX - 1)))) return rc;
+  } else {
+    const long long need = static_cast<long long>(st.nbr_total);
+    const long long cap = std::max(need + need / 2 + 4096, 2 * c.nbr_cap);
+    if ((rc = alloc_neighbors(c, cap))) {
+      g_last_error = "neighbor list of " + std::to_string(cap) + " words: " + g_last_error;
+      return FLUID_HIP_ECAPACITY;
+    }
+  }
+  drop_graph(c);
+  ++c.grow_events;
+  return FLUID_HIP_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+using fluidhip::AdjRecord;
+using fluidhip::SubTape;
+
+void free_tape(SubTape& t) {
+  for (float4** p : {&t.X, &t.PI, &t.VC, &t.P, &t.SQ}) dfree(*p);
+  dfree(t.M);
+  dfree(t.RM);
+  dfree(t.noff);
+  dfree(t.nwords);
+  dfree(t.ncount);
+  dfree(t.nbr);
+  t.nbr_cap = 0;
+  t.Pk.clear();
+  t.SQk.clear();
+  t.Mk.clear();
+}
+
+void free_adj(Ctx& c) {
+  if (!c.adj) return;
+  AdjRecord& a = *c.adj;
+  for (auto* p : a.ck_pos) hipFree(p);
+  for (auto* p : a.ck_vel) hipFree(p);
+  dfree(a.fin_pos);
+  dfree(a.fin_vel);
+  for (auto& t : a.tapes) free_tape(t);
+  auto& b = a.buf;
+  for (float4** p : {&b.GX, &b.GV, &b.U, &b.PB, &b.XB, &b.W, &b.LB}) dfree(*p);
+  dfree(b.RB);
+  dfree(b.gacc);
+  dfree(b.stage);
+  delete c.adj;
+  c.adj = nullptr;
+}
+
+void invalidate_record(Ctx& c) {
+  if (c.adj) c.adj->valid = false;
+}
+
+int alloc_tape(Ctx& c, SubTape& t) {
+  const size_t n = static_cast<size_t>(std::max(c.n, 1));
+  const size_t I = static_cast<size_t>(c.iterations);
+  HIP_TRY(dalloc(&t.X, n), "hipMalloc(tape)");
+  HIP_TRY(dalloc(&t.PI, n), "hipMalloc(tape)");
+  HIP_TRY(dalloc(&t.VC, n), "hipMalloc(tape)");
+  HIP_TRY(dalloc(&t.P, I * n), "hipMalloc(tape iterates)");
+  HIP_TRY(dalloc(&t.SQ, I * n), "hipMalloc(tape sums)");
+  HIP_TRY(dalloc(&t.M, I * n), "hipMalloc(tape masks)");
+  HIP_TRY(dalloc(&t.RM, n), "hipMalloc(tape masks)");
+  HIP_TRY(dalloc(&t.noff, n), "hipMalloc(tape lists)");
+  HIP_TRY(dalloc(&t.nwords, n), "hipMalloc(tape lists)");
+  HIP_TRY(dalloc(&t.ncount, n), "hipMalloc(tape lists)");
+  t.Pk.resize(I);
+  t.SQk.resize(I);
+  t.Mk.resize(I);
+  for (size_t k = 0; k < I; ++k) {
+    t.Pk[k] = t.P + k * n;
+    t.SQk[k] = t.SQ + k * n;
+    t.Mk[k] = t.M + k * n;
+  }
+  return FLUID_HIP_OK;
+}
+
+// One substep with its recording (replayed after capacity growth like any
+// other substep), then a copy of the input positions, the committed state
+// and the neighbor lists into the tape.
+int run_recorded(Ctx& c, SubTape& t) {
+  const fluidhip::SubstepRec rec{t.Pk.data(), t.SQk.data(), t.Mk.data(), t.RM};
+  DevStatus st{};
+  int rc;
+  for (int attempt = 0;; ++attempt) {
+    if (attempt >= 16) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
+    HIP_TRY(fluidhip::enqueue_reset_flags(c), "reset flags");
+    hipError_t e = fluidhip::enqueue_substep(c, &rec);
+    if (e != hipSuccess) return hip_fail(e, "recorded substep launch");
+    if ((rc = read_status(c, st))) return rc;
+    if (!(st.flags & fluidhip::kFlagFatal)) break;
+    if (st.flags & fluidhip::kFlagRange)
+      return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
+    if ((rc = grow_tables(c, st))) return rc;
+  }
+  const size_t n = static_cast<size_t>(c.n);
+  const long long words = static_cast<long long>(st.nbr_total);
+  if (words > t.nbr_cap) {
+    dfree(t.nbr);
+    const long long cap = words + words / 4 + 1024;
+    HIP_TRY(dalloc(&t.nbr, static_cast<size_t>(cap)), "hipMalloc(tape neighbor lists)");
+    t.nbr_cap = cap;
+  }
+  hipStream_t s = c.stream;
+  const auto D2D = hipMemcpyDeviceToDevice;
+  HIP_TRY(hipMemcpyAsync(t.X, c.posB, sizeof(float4) * n, D2D, s), "tape copy");
+  HIP_TRY(hipMemcpyAsync(t.PI, c.posA, sizeof(float4) * n, D2D, s), "tape copy");
+  if (c.xsph) HIP_TRY(hipMemcpyAsync(t.VC, c.velB, sizeof(float4) * n, D2D, s), "tape copy");
+  HIP_TRY(hipMemcpyAsync(t.noff, c.noff, sizeof(long long) * n, D2D, s), "tape copy");
+  HIP_TRY(hipMemcpyAsync(t.nwords, c.nwords, sizeof(int) * n, D2D, s), "tape copy");
+  HIP_TRY(hipMemcpyAsync(t.ncount, c.ncount, sizeof(int) * n, D2D, s), "tape copy");
+  if (words > 0)
+    HIP_TRY(hipMemcpyAsync(t.nbr, c.nbr, sizeof(int) * static_cast<size_t>(words), D2D, s),
+            "tape copy");
+  c.time += c.params[0].dt;
+  c.stepped = true;
+  return FLUID_HIP_OK;
+}
+
+int save_state(Ctx& c, float4* pos, float4* vel) {
+  const size_t bytes = sizeof(float4) * static_cast<size_t>(c.n);
+  if (!bytes) return FLUID_HIP_OK;
+  HIP_TRY(hipMemcpyAsync(pos, c.posA, bytes, hipMemcpyDeviceToDevice, c.stream), "checkpoint");
+  HIP_TRY(hipMemcpyAsync(vel, c.velA, bytes, hipMemcpyDeviceToDevice, c.stream), "checkpoint");
+  return FLUID_HIP_OK;
+}
+
+int load_state(Ctx& c, const float4* pos, const float4* vel) {
+  const size_t bytes = sizeof(float4) * static_cast<size_t>(c.n);
+  if (!bytes) return FLUID_HIP_OK;
+  HIP_TRY(hipMemcpyAsync(c.posA, pos, bytes, hipMemcpyDeviceToDevice, c.stream), "restore");
+  HIP_TRY(hipMemcpyAsync(c.velA, vel, bytes, hipMemcpyDeviceToDevice, c.stream), "restore");
+  return FLUID_HIP_OK;
+}
+
+long long tape_bytes(const Ctx& c) {
+  if (!c.adj) return 0;
+  const long long n = c.n, I = c.iterations;
+  long long total = 0;
+  for (const auto& t : c.adj->tapes)
+    total += n * (16 * 3 + 32 * I + 4 * I + 4 + 16) + 4 * t.nbr_cap;
+  total += static_cast<long long>(c.adj->ck_pos.size()) * 32 * n;
+  return total;
 }
 
 }  // namespace
@@ -490,6 +649,7 @@ int fluid_hip_set_params(fluid_hip_ctx* ctx, const fluid_hip_params* params) {
   const bool had_omega = c.omega != nullptr;
   c.params.assign(params, params + c.n_scenes);
   compute_consts(c);
+  invalidate_record(c);
   if (c.vort && !had_omega) HIP_TRY(dalloc(&c.omega, std::max(c.n, 1)), "hipMalloc omega");
   auto tables = scene_tables(c);
   HIP_TRY(hipMemcpyAsync(c.planes, tables.data(), sizeof(fluidhip::ScenePlanes) * c.n_scenes,
@@ -535,6 +695,7 @@ int fluid_hip_upload(fluid_hip_ctx* ctx, const float* pos, const float* vel, flo
   c.time = time;
   c.uploaded = true;
   c.stepped = false;
+  invalidate_record(c);
   return FLUID_HIP_OK;
 }
 
@@ -565,6 +726,7 @@ int fluid_hip_step_async(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) 
   if ((rc = queue_steps(c, nsteps, use_graph != 0))) return rc;
   for (int s = 0; s < nsteps; ++s) c.time += c.params[0].dt;
   if (nsteps > 0) c.stepped = true;
+  if (nsteps > 0) invalidate_record(c);
   return FLUID_HIP_OK;
 }
 
@@ -587,6 +749,7 @@ int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
   Ctx& c = ctx->c;
   if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
   if (nsteps < 0) return fail(FLUID_HIP_EINVAL, "nsteps < 0");
+  if (nsteps > 0) invalidate_record(c);
   int remaining = nsteps;
   for (int attempt = 0; remaining > 0 && attempt < 16; ++attempt) {
     HIP_TRY(fluidhip::enqueue_reset_flags(c), "reset flags");
@@ -605,22 +768,7 @@ int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
       return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
     // Capacity: grow to the reported need and resume from the first step that
     // did not complete (the state of a failed step is left untouched).
-    if (st.flags & fluidhip::kFlagCells) {
-      const long long need = static_cast<long long>(st.need_cells);
-      const long long cap = std::max(need + need / 2 + 4096, 2 * c.cell_cap);
-      if (need + 4096 >= INT_MAX)
-        return fail(FLUID_HIP_ECAPACITY, "cell table would exceed 2^31 cells");
-      if ((rc = alloc_cells(c, std::min<long long>(cap, INT_MAX - 1)))) return rc;
-    } else {
-      const long long need = static_cast<long long>(st.nbr_total);
-      const long long cap = std::max(need + need / 2 + 4096, 2 * c.nbr_cap);
-      if ((rc = alloc_neighbors(c, cap))) {
-        g_last_error = "neighbor list of " + std::to_string(cap) + " words: " + g_last_error;
-        return FLUID_HIP_ECAPACITY;
-      }
-    }
-    drop_graph(c);
-    ++c.grow_events;
+    if ((rc = grow_tables(c, st))) return rc;
   }
   if (remaining > 0) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
   return FLUID_HIP_OK;
@@ -744,7 +892,8 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
   if (rc) return rc;
   Ctx& c = ctx->c;
   if (!out || n < 0) return fail(FLUID_HIP_EINVAL, "bad output");
-  int64_t v[6] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events};
+  int64_t v[8] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events, tape_bytes(c),
+                  c.adj ? c.adj->recomputed : 0};
   if (c.stepped && c.n > 0) {
     std::vector<int> cnt(c.n);
     HIP_TRY(hipMemcpy(cnt.data(), c.ncount, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read counts");
@@ -756,7 +905,7 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
     HIP_TRY(hipMemcpy(&cells, c.total_cells, sizeof(int), hipMemcpyDeviceToHost), "read cells");
     v[2] = cells;
   }
-  for (int i = 0; i < n && i < 6; ++i) out[i] = v[i];
+  for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
   return FLUID_HIP_OK;
 }
 
@@ -782,6 +931,147 @@ int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total
   auto it = c.prof.acc.find(kernel);
   if (total_ms) *total_ms = it == c.prof.acc.end() ? 0.0 : it->second.first;
   if (launches) *launches = it == c.prof.acc.end() ? 0 : it->second.second;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_forward_record(fluid_hip_ctx* ctx, int32_t nsteps, int32_t checkpoint_every) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  if (nsteps < 0 || checkpoint_every < 1)
+    return fail(FLUID_HIP_EINVAL, "nsteps < 0 or checkpoint_every < 1");
+  if (c.vort)
+    return fail(FLUID_HIP_EUNSUPPORTED, "the adjoint does not cover vorticity confinement");
+  if (c.iterations < 1)
+    return fail(FLUID_HIP_EUNSUPPORTED, "the adjoint needs solver_iterations >= 1");
+  if (!c.adj) {
+    c.adj = new (std::nothrow) AdjRecord();
+    if (!c.adj) return fail(FLUID_HIP_EHIP, "out of host memory");
+    auto& b = c.adj->buf;
+    const size_t n = static_cast<size_t>(std::max(c.n, 1));
+    for (float4** p : {&b.GX, &b.GV, &b.U, &b.PB, &b.XB, &b.W, &b.LB})
+      HIP_TRY(dalloc(p, n), "hipMalloc(adjoint buffers)");
+    HIP_TRY(dalloc(&b.RB, n), "hipMalloc(adjoint buffers)");
+    HIP_TRY(dalloc(&b.gacc, 3 * static_cast<size_t>(c.n_scenes)), "hipMalloc(adjoint buffers)");
+    HIP_TRY(dalloc(&b.stage, 6 * n), "hipMalloc(adjoint buffers)");
+  }
+  AdjRecord& a = *c.adj;
+  a.valid = false;
+  const int nseg = nsteps == 0 ? 0 : (nsteps + checkpoint_every - 1) / checkpoint_every;
+  const int ntapes = std::min(nsteps, checkpoint_every);
+  const size_t n = static_cast<size_t>(std::max(c.n, 1));
+  // checkpoints: one per segment; tapes: one per substep of a segment
+  while (static_cast<int>(a.ck_pos.size()) > nseg) {
+    hipFree(a.ck_pos.back());
+    hipFree(a.ck_vel.back());
+    a.ck_pos.pop_back();
+    a.ck_vel.pop_back();
+  }
+  while (static_cast<int>(a.ck_pos.size()) < nseg) {
+    float4 *p = nullptr, *v = nullptr;
+    HIP_TRY(dalloc(&p, n), "hipMalloc(checkpoint)");
+    a.ck_pos.push_back(p);
+    HIP_TRY(dalloc(&v, n), "hipMalloc(checkpoint)");
+    a.ck_vel.push_back(v);
+  }
+  a.ck_time.assign(nseg, 0.0f);
+  const int iters_of_tapes = a.tapes.empty() ? -1 : static_cast<int>(a.tapes[0].Pk.size());
+  if (iters_of_tapes != c.iterations) {
+    for (auto& t : a.tapes) free_tape(t);
+    a.tapes.clear();
+  }
+  while (static_cast<int>(a.tapes.size()) > ntapes) {
+    free_tape(a.tapes.back());
+    a.tapes.pop_back();
+  }
+  while (static_cast<int>(a.tapes.size()) < ntapes) {
+    a.tapes.emplace_back();
+    if ((rc = alloc_tape(c, a.tapes.back()))) return rc;
+  }
+  if (!a.fin_pos) {
+    HIP_TRY(dalloc(&a.fin_pos, n), "hipMalloc(final state)");
+    HIP_TRY(dalloc(&a.fin_vel, n), "hipMalloc(final state)");
+  }
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int s0 = sg * checkpoint_every;
+    const int len = std::min(checkpoint_every, nsteps - s0);
+    if ((rc = save_state(c, a.ck_pos[sg], a.ck_vel[sg]))) return rc;
+    a.ck_time[sg] = c.time;
+    if (sg + 1 < nseg) {
+      // plain substeps; only the last segment stays recorded
+      if ((rc = fluid_hip_step(ctx, len, 0))) return rc;
+    } else {
+      for (int j = 0; j < len; ++j)
+        if ((rc = run_recorded(c, a.tapes[j]))) return rc;
+    }
+  }
+  if ((rc = save_state(c, a.fin_pos, a.fin_vel))) return rc;
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  a.fin_time = c.time;
+  a.nsteps = nsteps;
+  a.every = checkpoint_every;
+  a.nseg = nseg;
+  a.tape_seg = nseg - 1;
+  a.recomputed = 0;
+  a.valid = true;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel, float* dpos0,
+                       float* dvel0, float* dgravity) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.adj || !c.adj->valid)
+    return fail(FLUID_HIP_ESTATE, "no recorded rollout (fluid_hip_forward_record)");
+  AdjRecord& a = *c.adj;
+  auto& b = a.buf;
+  const size_t n = static_cast<size_t>(c.n);
+  const size_t bytes = sizeof(float) * 3 * n;
+  float* sp = b.stage;
+  float* sv = b.stage + 3 * std::max<size_t>(n, 1);
+  if (n > 0) {
+    if (dpos) HIP_TRY(hipMemcpyAsync(sp, dpos, bytes, hipMemcpyHostToDevice, c.stream), "dpos");
+    if (dvel) HIP_TRY(hipMemcpyAsync(sv, dvel, bytes, hipMemcpyHostToDevice, c.stream), "dvel");
+  }
+  HIP_TRY(fluidhip::enqueue_adjoint_seed(c, dpos ? sp : nullptr, dvel ? sv : nullptr, b),
+          "adjoint seed");
+  a.recomputed = 0;
+  for (int sg = a.nseg - 1; sg >= 0; --sg) {
+    const int s0 = sg * a.every;
+    const int len = std::min(a.every, a.nsteps - s0);
+    if (a.tape_seg != sg) {
+      // recompute the segment from its checkpoint, recording every substep
+      if ((rc = load_state(c, a.ck_pos[sg], a.ck_vel[sg]))) return rc;
+      c.time = a.ck_time[sg];
+      for (int j = 0; j < len; ++j) {
+        if ((rc = run_recorded(c, a.tapes[j]))) {
+          a.tape_seg = -1;
+          return rc;
+        }
+      }
+      a.tape_seg = sg;
+      a.recomputed += len;
+    }
+    for (int j = len - 1; j >= 0; --j)
+      HIP_TRY(fluidhip::enqueue_adjoint_substep(c, a.tapes[j], b), "adjoint substep");
+  }
+  HIP_TRY(fluidhip::enqueue_adjoint_result(c, sp, sv, b), "adjoint result");
+  if (n > 0) {
+    if (dpos0) HIP_TRY(hipMemcpyAsync(dpos0, sp, bytes, hipMemcpyDeviceToHost, c.stream), "dpos0");
+    if (dvel0) HIP_TRY(hipMemcpyAsync(dvel0, sv, bytes, hipMemcpyDeviceToHost, c.stream), "dvel0");
+  }
+  std::vector<double> g(3 * static_cast<size_t>(c.n_scenes));
+  HIP_TRY(hipMemcpyAsync(g.data(), b.gacc, sizeof(double) * g.size(), hipMemcpyDeviceToHost,
+                         c.stream),
+          "dgravity");
+  // leave the context at the recorded final state
+  if ((rc = load_state(c, a.fin_pos, a.fin_vel))) return rc;
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  c.time = a.fin_time;
+  if (dgravity)
+    for (size_t i = 0; i < g.size(); ++i) dgravity[i] = static_cast<float>(g[i]);
   return FLUID_HIP_OK;
 }
 

@@ -101,10 +101,78 @@ struct Ctx {
   hipGraphExec_t graph_exec = nullptr;
 
   Prof prof;
+
+  // differentiable rollout (created by the first recording)
+  struct AdjRecord* adj = nullptr;
 };
 
-// Queues one substep on ctx.stream (kernels in pbf_kernels.hip).
-hipError_t enqueue_substep(Ctx& c);
+// Where a recorded substep writes what its adjoint needs (device arrays in
+// that substep's sorted slot order, one entry per constraint iteration).
+struct SubstepRec {
+  float4* const* P;    // iterate p^k of the constraint loop, w = lambda^k
+  float4* const* SQ;   // gradient sum S (xyz) and denominator sum Q (w)
+  unsigned* const* M;  // planes that pushed p^k + delta
+  unsigned* RM;        // restitution: planes in contact | approached << 8
+};
+
+// Queues one substep on ctx.stream (kernels in pbf_kernels.hip); with `rec`
+// the iterates and branch decisions are also written to the recording.
+hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec = nullptr);
+
+// What the adjoint of one substep reads (device arrays, that substep's slot
+// order; P, SQ and M hold `iterations` consecutive blocks of n entries).
+struct SubTape {
+  float4* X = nullptr;    // input positions, w = original id
+  float4* PI = nullptr;   // committed positions p^I
+  float4* VC = nullptr;   // commit velocity (xyz) and rho of the last lambda pass (w)
+  float4* P = nullptr;    // p^k, w = lambda^k
+  float4* SQ = nullptr;   // gradient sum S (xyz), Q (w)
+  unsigned* M = nullptr;  // planes that pushed p^k + delta
+  unsigned* RM = nullptr; // restitution decisions
+  long long* noff = nullptr;
+  int* nwords = nullptr;
+  int* ncount = nullptr;
+  int* nbr = nullptr;     // the substep's run-length-encoded lists
+  long long nbr_cap = 0;
+  std::vector<float4*> Pk, SQk;
+  std::vector<unsigned*> Mk;
+};
+
+// Adjoint state carried between substeps: position and velocity adjoints in
+// original particle order, per-slot work arrays, per-scene g-bar (double).
+struct AdjBuffers {
+  float4* GX = nullptr;
+  float4* GV = nullptr;
+  float4* U = nullptr;   // adjoint of the XSPH output velocity
+  float4* PB = nullptr;  // adjoint of the current iterate
+  float4* XB = nullptr;  // commit's share of the input-position adjoint
+  float4* W = nullptr;   // delta-bar / rho0
+  float4* LB = nullptr;  // (Q-bar S, Q-bar)
+  float* RB = nullptr;   // rho-bar
+  double* gacc = nullptr;
+  float* stage = nullptr;  // 6n floats: SoA transfers
+};
+
+// A recorded rollout: the state before every segment of `every` substeps,
+// the final state, and the recordings of one segment's substeps.
+struct AdjRecord {
+  bool valid = false;
+  int nsteps = 0, every = 0, nseg = 0;
+  int tape_seg = -1;  // segment the tapes currently hold
+  std::vector<float4*> ck_pos, ck_vel;
+  std::vector<float> ck_time;
+  float4* fin_pos = nullptr;
+  float4* fin_vel = nullptr;
+  float fin_time = 0.0f;
+  std::vector<SubTape> tapes;
+  AdjBuffers buf;
+  int recomputed = 0;  // substeps re-run by the last backward
+};
+
+hipError_t enqueue_adjoint_seed(Ctx& c, const float* d_dpos, const float* d_dvel,
+                                const AdjBuffers& b);
+hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b);
+hipError_t enqueue_adjoint_result(Ctx& c, float* d_dpos, float* d_dvel, const AdjBuffers& b);
 hipError_t enqueue_upload(Ctx& c, const float* d_pos, const float* d_vel);
 hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel);
 hipError_t enqueue_reset_flags(Ctx& c);

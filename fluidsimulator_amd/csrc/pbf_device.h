@@ -165,4 +165,93 @@ __device__ __forceinline__ float3 ld3(const float4* p, int i) {
   return make_float3(f[0], f[1], f[2]);
 }
 
+// Sequential push-out along each plane normal (core.cpp:372-389).  Returns
+// the planes that pushed (bit p), which fixes the Jacobian of the
+// projection: I - n n^T per pushing plane, applied in plane order.
+__device__ __forceinline__ unsigned project_planes(float& qx, float& qy, float& qz,
+                                                   const ScenePlanes& sp) {
+  unsigned mask = 0;
+  for (int p = 0; p < sp.n_planes; ++p) {
+    const float4 pl = sp.planes[p];
+    const float sd = pl.x * qx + pl.y * qy + pl.z * qz - pl.w;
+    const float pen = -sd;
+    if (pen > 0.0f) {
+      qx += pl.x * pen;
+      qy += pl.y * pen;
+      qz += pl.z * pen;
+      mask |= 1u << p;
+    }
+  }
+  return mask;
+}
+
+// Plane restitution and friction (core.cpp:573-612).  Returns the planes in
+// contact (bit p) and, among those, the ones the velocity approached
+// (bit 8 + p): with them the map is linear, v -> (1-f) v + (a - (1-f)) (n.v) n
+// with a = -e when approaching and 1 otherwise.
+__device__ __forceinline__ unsigned restitute(float& vx, float& vy, float& vz, float px, float py,
+                                              float pz, const ScenePlanes& sp) {
+  if (!sp.restitution_on) return 0;
+  unsigned mask = 0;
+  for (int p = 0; p < sp.n_planes; ++p) {
+    const float4 pl = sp.planes[p];
+    const float sd = pl.x * px + pl.y * py + pl.z * pz - pl.w;
+    if (sd <= 0.0f) {
+      const float vn = pl.x * vx + pl.y * vy + pl.z * vz;
+      float vn_new = vn;
+      mask |= 1u << p;
+      if (vn < 0.0f) {
+        vn_new = -sp.restitution * vn;
+        mask |= 1u << (8 + p);
+      }
+      const float tx = vx - vn * pl.x;
+      const float ty = vy - vn * pl.y;
+      const float tz = vz - vn * pl.z;
+      const float scale = 1.0f - sp.friction;
+      vx = tx * scale + vn_new * pl.x;
+      vy = ty * scale + vn_new * pl.y;
+      vz = tz * scale + vn_new * pl.z;
+    }
+  }
+  return mask;
+}
+
+// Run-length-encoded neighbor lists in slot space: a word t >= 0 is the
+// single neighbor t; a word with the top bit set starts a run of
+// consecutive slots (t & 0x7fffffff) .. + next word - 1.
+struct NbrView {
+  const long long* __restrict__ off;
+  const int* __restrict__ nw;
+  const int* __restrict__ list;
+  const int* __restrict__ count;
+};
+
+// Calls f(t, load(t)) for each neighbor slot t of slot s, in list order.
+// Runs are walked four slots at a time with the four loads issued before
+// any use, which keeps several memory requests in flight per lane; the
+// callbacks still run strictly in list order, so every sum accumulates in
+// the reference's order.
+template <typename Load, typename F>
+__device__ __forceinline__ void for_each_neighbor(const NbrView& v, int s, Load&& load, F&& f) {
+  const int* __restrict__ L = v.list + v.off[s];
+  const int nw = v.nw[s];
+  for (int w = 0; w < nw;) {
+    const int e = L[w++];
+    int t = e & 0x7fffffff;
+    int end = t + 1;
+    if (e < 0) end = t + L[w++];
+    for (; t + 4 <= end; t += 4) {
+      const auto a = load(t);
+      const auto b = load(t + 1);
+      const auto c = load(t + 2);
+      const auto d = load(t + 3);
+      f(t, a);
+      f(t + 1, b);
+      f(t + 2, c);
+      f(t + 3, d);
+    }
+    for (; t < end; ++t) f(t, load(t));
+  }
+}
+
 }  // namespace fluidhip

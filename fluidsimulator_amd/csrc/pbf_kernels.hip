@@ -92,43 +92,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total) {
   return r;
 }
 
-__device__ __forceinline__ void project_planes(float& qx, float& qy, float& qz,
-                                               const ScenePlanes& sp) {
-  // core.cpp:372-389: sequential push-out along each plane normal.
-  for (int p = 0; p < sp.n_planes; ++p) {
-    const float4 pl = sp.planes[p];
-    const float sd = pl.x * qx + pl.y * qy + pl.z * qz - pl.w;
-    const float pen = -sd;
-    if (pen > 0.0f) {
-      qx += pl.x * pen;
-      qy += pl.y * pen;
-      qz += pl.z * pen;
-    }
-  }
-}
-
-__device__ __forceinline__ void restitute(float& vx, float& vy, float& vz, float px, float py,
-                                          float pz, const ScenePlanes& sp) {
-  // core.cpp:573-612
-  if (!sp.restitution_on) return;
-  for (int p = 0; p < sp.n_planes; ++p) {
-    const float4 pl = sp.planes[p];
-    const float sd = pl.x * px + pl.y * py + pl.z * pz - pl.w;
-    if (sd <= 0.0f) {
-      const float vn = pl.x * vx + pl.y * vy + pl.z * vz;
-      float vn_new = vn;
-      if (vn < 0.0f) vn_new = -sp.restitution * vn;
-      const float tx = vx - vn * pl.x;
-      const float ty = vy - vn * pl.y;
-      const float tz = vz - vn * pl.z;
-      const float scale = 1.0f - sp.friction;
-      vx = tx * scale + vn_new * pl.x;
-      vy = ty * scale + vn_new * pl.y;
-      vz = tz * scale + vn_new * pl.z;
-    }
-  }
-}
-
 // ---------------------------------------------------------------- wavefront
 
 // Lane-to-lane broadcasts.  They must run on the whole wavefront: the
@@ -857,41 +820,6 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   }
 }
 
-struct NbrView {
-  const long long* __restrict__ off;
-  const int* __restrict__ nw;
-  const int* __restrict__ list;
-  const int* __restrict__ count;
-};
-
-// Calls f(t, load(t)) for each neighbor slot t of slot s, in list order.
-// Runs are walked four slots at a time with the four loads issued before
-// any use, which keeps several memory requests in flight per lane; the
-// callbacks still run strictly in list order, so every sum accumulates in
-// the reference's order.
-template <typename Load, typename F>
-__device__ __forceinline__ void for_each_neighbor(const NbrView& v, int s, Load&& load, F&& f) {
-  const int* __restrict__ L = v.list + v.off[s];
-  const int nw = v.nw[s];
-  for (int w = 0; w < nw;) {
-    const int e = L[w++];
-    int t = e & 0x7fffffff;
-    int end = t + 1;
-    if (e < 0) end = t + L[w++];
-    for (; t + 4 <= end; t += 4) {
-      const auto a = load(t);
-      const auto b = load(t + 1);
-      const auto c = load(t + 2);
-      const auto d = load(t + 3);
-      f(t, a);
-      f(t + 1, b);
-      f(t + 2, c);
-      f(t + 3, d);
-    }
-    for (; t < end; ++t) f(t, load(t));
-  }
-}
-
 // A lane's position in its own list during a cooperative walk.
 struct ListCursor {
   const int* __restrict__ L;
@@ -1038,10 +966,13 @@ __device__ __forceinline__ bool tile_all(bool ok) { return __ballot(ok) == ~0ull
 // reference adds it.  A tile whose candidates all sit at one point gives
 // every lane one fixed pair geometry, computed once for the tile.
 
-// lambda pass (core.cpp:281-329).  Writes lambda into pred[s].w.
+// lambda pass (core.cpp:281-329).  Writes lambda into pred[s].w; with
+// sq_out (a recorded substep) also the gradient sum S (xyz) and the
+// denominator sum Q (w), lambda = -C / (Q + eps).
 __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restrict__ P,
                                                  NbrView nv, CellView cv,
                                                  float* __restrict__ rho_out, int write_rho,
+                                                 float4* __restrict__ sq_out,
                                                  const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
@@ -1117,16 +1048,20 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
   sg2 += gix * gix + giy * giy + giz * giz;
   reinterpret_cast<float*>(P + s)[3] = -C / (sg2 + c.eps);
   if (write_rho) rho_out[s] = rho;
+  if (sq_out) sq_out[s] = make_float4(gsx, gsy, gsz, sg2);
 }
 
 // delta pass with plane projection and apply (core.cpp:334-407).  kLast
 // also commits (core.cpp:414-421) into posA/vout; kSolve=false is the
 // zero-iteration commit.  kRest fuses plane restitution (core.cpp:573-612).
+// A recorded substep also stores the planes that pushed (pmask) and the
+// restitution decisions (rmask).
 template <bool kSolve, bool kLast, bool kRest>
 __global__ __launch_bounds__(1024) void k_delta(
     StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext, NbrView nv,
     CellView cv, const ScenePlanes* __restrict__ sp, const float4* __restrict__ posB,
     float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
+    unsigned* __restrict__ pmask, unsigned* __restrict__ rmask,
     const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
@@ -1193,13 +1128,15 @@ __global__ __launch_bounds__(1024) void k_delta(
     ddy *= c.inv_density;
     ddz *= c.inv_density;
     const ScenePlanes& sc = sp[scene_of(s, cv.seg, c.n_scenes)];
+    unsigned pushed = 0;
     if (sc.n_planes > 0) {
       float qx = pi.x + ddx, qy = pi.y + ddy, qz = pi.z + ddz;
-      project_planes(qx, qy, qz, sc);
+      pushed = project_planes(qx, qy, qz, sc);
       ddx = qx - pi.x;
       ddy = qy - pi.y;
       ddz = qz - pi.z;
     }
+    if (pmask) pmask[s] = pushed;
     nx = pi.x + ddx;  // core.cpp:404-406
     ny = pi.y + ddy;
     nz = pi.z + ddz;
@@ -1214,7 +1151,10 @@ __global__ __launch_bounds__(1024) void k_delta(
   float vy = (ny - x.y) / c.dt;
   float vz = (nz - x.z) / c.dt;
   posA[s] = make_float4(nx, ny, nz, x.w);
-  if (kRest) restitute(vx, vy, vz, nx, ny, nz, sp[scene_of(s, cv.seg, c.n_scenes)]);
+  if (kRest) {
+    const unsigned rm = restitute(vx, vy, vz, nx, ny, nz, sp[scene_of(s, cv.seg, c.n_scenes)]);
+    if (rmask) rmask[s] = rm;
+  }
   vout[s] = make_float4(vx, vy, vz, kSolve ? rho[s] : 0.0f);
 }
 
@@ -1224,6 +1164,7 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
                                                const float4* __restrict__ vin,
                                                float4* __restrict__ vout, NbrView nv,
                                                CellView cv, const ScenePlanes* __restrict__ sp,
+                                               unsigned* __restrict__ rmask,
                                                const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
@@ -1281,7 +1222,11 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
   float vx = vi.x + c.visc_c * ax;
   float vy = vi.y + c.visc_c * ay;
   float vz = vi.z + c.visc_c * az;
-  if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, cv.seg, c.n_scenes)]);
+  if (kRest) {
+    const unsigned rm =
+        restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[scene_of(s, cv.seg, c.n_scenes)]);
+    if (rmask) rmask[s] = rm;
+  }
   vout[s] = make_float4(vx, vy, vz, 0.0f);
 }
 
@@ -1477,8 +1422,12 @@ hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel) {
   return hipGetLastError();
 }
 
-hipError_t enqueue_substep(Ctx& c) {
+hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (c.n == 0) return hipSuccess;
+  if (rec && c.iterations < 1) return hipErrorInvalidValue;
+  // iterate k of the constraint loop: the Jacobi ping-pong pair, or the
+  // recording's own array per iterate
+  auto PK = [&](int k) { return rec ? rec->P[k] : c.pred[k & 1]; };
   Launcher L{c};
   const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg};
   const StepConsts k = c.consts;
@@ -1519,11 +1468,11 @@ hipError_t enqueue_substep(Ctx& c) {
   }));
   FH_TRY(L("reorder", [&] {
     k_reorder<<<nb, bs, 0, st>>>(k, c.perm, c.sid, c.skey, c.start, c.seg, c.planes, c.posA,
-                                 c.velA, c.posB, c.velB, c.pred[0], c.cellc, c.slot_cell,
+                                 c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
   FH_TRY(L("neighbors", [&] {
-    k_neighbors<<<nb, bs, 0, st>>>(k, c.pred[0], cv, c.noff, c.nwords,
+    k_neighbors<<<nb, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
   }));
 
@@ -1534,25 +1483,28 @@ hipError_t enqueue_substep(Ctx& c) {
   const bool rest_in_delta = c.restitution_any && !c.xsph && !c.vort;
   const int I = c.iterations;
   for (int it = 0; it < I; ++it) {
-    float4* P = c.pred[it & 1];
-    float4* Pn = c.pred[(it + 1) & 1];
     const bool last = (it + 1 == I);
+    float4* P = PK(it);
+    float4* Pn = last ? nullptr : PK(it + 1);
+    float4* sq = rec ? rec->SQ[it] : nullptr;
+    unsigned* pm = rec ? rec->M[it] : nullptr;
+    unsigned* rm = rec ? rec->RM : nullptr;
     FH_TRY(L("lambda", [&] {
-      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, c.flags);
+      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
     }));
     FH_TRY(L("delta", [&] {
       if (!last)
         k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
-                                                       c.flags);
+                                                       pm, nullptr, c.flags);
       else if (rest_in_delta)
         k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                      c.planes, c.posB, c.posA, v1, c.rho,
-                                                     c.flags);
+                                                     pm, rm, c.flags);
       else
         k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
-                                                      c.flags);
+                                                      pm, nullptr, c.flags);
     }));
   }
   if (I == 0) {
@@ -1560,11 +1512,11 @@ hipError_t enqueue_substep(Ctx& c) {
       if (rest_in_delta)
         k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
-                                                      c.flags);
+                                                      nullptr, nullptr, c.flags);
       else
         k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
-                                                       c.flags);
+                                                       nullptr, nullptr, c.flags);
     }));
   }
   if (c.xsph) {
@@ -1572,10 +1524,10 @@ hipError_t enqueue_substep(Ctx& c) {
     FH_TRY(L("xsph", [&] {
       if (rest)
         k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
-                                        c.planes, c.flags);
+                                        c.planes, rec ? rec->RM : nullptr, c.flags);
       else
         k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
-                                         c.planes, c.flags);
+                                         c.planes, nullptr, c.flags);
     }));
   }
   if (c.vort) {

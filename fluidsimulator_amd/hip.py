@@ -80,8 +80,12 @@ _SIGS = {
                                          C.POINTER(C.c_int64)]),
 }
 
-# Entry points added by later components (adjoint); bound when exported.
-_OPTIONAL_SIGS: dict = {}
+# Differentiable rollout.
+_OPTIONAL_SIGS: dict = {
+    "fluid_hip_forward_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "fluid_hip_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+}
 
 
 def exported_symbols():
@@ -263,11 +267,40 @@ class HipSolver:
 
     def stats(self):
         """Sizes of the last step: neighbor pairs, encoded list words, grid
-        cells, list capacity (words), cell capacity, capacity growths."""
-        out = (C.c_int64 * 6)()
-        _check(self.lib.fluid_hip_get_stats(self.h, out, 6))
-        keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events")
+        cells, list capacity (words), cell capacity, capacity growths; device
+        bytes of the rollout recording and substeps the last backward pass
+        recomputed."""
+        out = (C.c_int64 * 8)()
+        _check(self.lib.fluid_hip_get_stats(self.h, out, 8))
+        keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events",
+                "record_bytes", "recomputed")
         return dict(zip(keys, [int(x) for x in out]))
+
+    def forward_record(self, nsteps, checkpoint_every=10):
+        """Run `nsteps` substeps keeping what the reverse pass needs (state
+        every `checkpoint_every` substeps, recordings of the last segment)."""
+        _check(self.lib.fluid_hip_forward_record(self.h, int(nsteps), int(checkpoint_every)))
+
+    def backward(self, dpos=None, dvel=None):
+        """Reverse pass through the recorded rollout.  dpos, dvel: dL/dx_T,
+        dL/dv_T as (3, n) arrays in original order (None = zero).  Returns
+        (dL/dx_0, dL/dv_0, dL/dg) with dL/dg of shape (n_scenes, 3)."""
+        n = self.n
+        def arg(a):
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(a, dtype=np.float32).reshape(3, n)
+            return a, a.ctypes.data_as(C.c_void_p)
+        kp, pp = arg(dpos)
+        kv, pv = arg(dvel)
+        dx0 = np.zeros((3, n), np.float32)
+        dv0 = np.zeros((3, n), np.float32)
+        dg = np.zeros((self.n_scenes, 3), np.float32)
+        _check(self.lib.fluid_hip_backward(self.h, pp, pv, dx0.ctypes.data_as(C.c_void_p),
+                                           dv0.ctypes.data_as(C.c_void_p),
+                                           dg.ctypes.data_as(C.c_void_p)))
+        del kp, kv
+        return dx0, dv0, dg
 
     def profile(self, enable=True):
         _check(self.lib.fluid_hip_profile(self.h, int(bool(enable))))

@@ -171,8 +171,41 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
 /* Sizes of the last step: out[0] neighbor pairs (decoded), out[1] encoded
  * list words, out[2] grid cells (all scenes), out[3] neighbor-list capacity
  * (words), out[4] cell-table capacity, out[5] capacity growths since the
- * context was created.  `n` entries are written (<= 6). */
+ * context was created, out[6] device bytes held by the rollout recording
+ * (checkpoints and tapes), out[7] substeps the last backward pass
+ * recomputed.  `n` entries are written (<= 8). */
 int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
+
+/* ---- differentiable rollout ---------------------------------------------- */
+
+/* No counterpart in the reference (its solver has no derivatives).  For
+ * inverse problems: gradients of a loss on the final state with respect to
+ * the initial positions, velocities and per-scene gravity.
+ *
+ * Runs `nsteps` substeps from the current state, as fluid_hip_step does, and
+ * keeps what the reverse pass needs: the state before every segment of
+ * `checkpoint_every` substeps, and the recording (iterates, lambdas, plane
+ * and restitution decisions, neighbor lists) of the last segment.  Leaves the
+ * context at the final state.  Memory: 32 B per particle per checkpoint, and
+ * per recorded substep about (96 + 36 * iterations) B per particle plus its
+ * neighbor lists.  Any later upload, step or set_params discards the record.
+ * Vorticity confinement and zero constraint iterations are refused
+ * (FLUID_HIP_EUNSUPPORTED). */
+int fluid_hip_forward_record(fluid_hip_ctx* ctx, int32_t nsteps, int32_t checkpoint_every);
+
+/* Reverse pass through the recorded rollout: the hand-written adjoint of
+ * every substep (predict, each lambda/delta iteration with s_corr and plane
+ * projection, commit, XSPH, restitution/friction) with each substep's
+ * neighbor lists frozen as the forward built them.  Segments other than the
+ * recorded one are recomputed from their checkpoints.
+ *   dpos, dvel    dL/dx_T and dL/dv_T, 3n floats SoA in original order (NULL:
+ *                 zero)
+ *   dpos0, dvel0  dL/dx_0, dL/dv_0 (3n SoA; NULL: not returned)
+ *   dgravity      dL/dg, 3 floats per scene (NULL: not returned)
+ * The context is left at the recorded final state; the record stays valid, so
+ * the pass can be repeated with other loss gradients. */
+int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel, float* dpos0,
+                       float* dvel0, float* dgravity);
 
 /* ---- timing -------------------------------------------------------------- */
 
