@@ -326,6 +326,18 @@ int read_status(Ctx& c, DevStatus& st) {
   return collect_profile(c);
 }
 
+// log2 of a brick edge (0 selects the default 4), or -1 when unsupported.
+int brick_shift_of(int edge) {
+  switch (edge) {
+    case 0: return 2;
+    case 1: return 0;
+    case 2: return 1;
+    case 4: return 2;
+    case 8: return 3;
+    default: return -1;
+  }
+}
+
 int check_ctx(fluid_hip_ctx* ctx) {
   if (!ctx) return fail(FLUID_HIP_EINVAL, "context is null");
   HIP_TRY(hipSetDevice(ctx->c.device), "hipSetDevice");
@@ -533,6 +545,10 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   const int bs = cfg.block_size ? cfg.block_size : 256;
   if (bs < 64 || bs > 1024 || (bs % 64) != 0)
     return fail(FLUID_HIP_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
+  if (cfg.wave_mode < 0 || cfg.wave_mode > 3)
+    return fail(FLUID_HIP_EINVAL, "wave_mode must be 0..3");
+  const int bshift = brick_shift_of(cfg.brick_edge);
+  if (bshift < 0) return fail(FLUID_HIP_EINVAL, "brick_edge must be 0, 1, 2, 4 or 8");
 
   auto* h = new (std::nothrow) fluid_hip_ctx();
   if (!h) return fail(FLUID_HIP_EINVAL, "out of host memory");
@@ -541,9 +557,8 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   c.block = bs;
   c.xcd_remap = cfg.xcd_remap ? 1 : 0;
   c.record_diag = cfg.record_diag != 0;
-  if (cfg.wave_mode < 0 || cfg.wave_mode > 3)
-    return fail(FLUID_HIP_EINVAL, "wave_mode must be 0..3");
   c.wave_mode = cfg.wave_mode ? cfg.wave_mode : 2;
+  c.brick_shift = bshift;
   c.n = static_cast<int>(total);
   c.n_scenes = n_scenes;
   c.params.assign(params, params + n_scenes);
@@ -880,9 +895,15 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc) {
     HIP_TRY(hipMemcpy(slot_keys, c.slot_key, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read keys");
   }
   if (desc) {
-    static_assert(sizeof(fluidhip::GridDesc) == 8 * sizeof(int32_t), "GridDesc layout");
-    HIP_TRY(hipMemcpy(desc, c.grid, sizeof(fluidhip::GridDesc) * c.n_scenes, hipMemcpyDeviceToHost),
+    std::vector<fluidhip::GridDesc> g(c.n_scenes);
+    HIP_TRY(hipMemcpy(g.data(), c.grid, sizeof(fluidhip::GridDesc) * c.n_scenes,
+                      hipMemcpyDeviceToHost),
             "read grid");
+    for (int s = 0; s < c.n_scenes; ++s) {
+      const int32_t v[8] = {g[s].ox, g[s].oy, g[s].oz, g[s].nbx, g[s].nby, g[s].nbz,
+                            g[s].cell_off, g[s].ncells};
+      std::memcpy(desc + 8 * s, v, sizeof(v));
+    }
   }
   return FLUID_HIP_OK;
 }
@@ -931,6 +952,147 @@ int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total
   auto it = c.prof.acc.find(kernel);
   if (total_ms) *total_ms = it == c.prof.acc.end() ? 0.0 : it->second.first;
   if (launches) *launches = it == c.prof.acc.end() ? 0 : it->second.second;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, fluid_hip_config* best,
+                       double* report) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  if (trial_steps < 1) return fail(FLUID_HIP_EINVAL, "trial_steps < 1");
+  struct Knobs {
+    int block, remap, wave, bshift;
+  };
+  const Knobs start{c.block, c.xcd_remap, c.wave_mode, c.brick_shift};
+  const size_t n = static_cast<size_t>(c.n);
+  const size_t soa = 3 * n;
+  invalidate_record(c);
+
+  // every trial starts from this state and is compared bitwise with the
+  // starting configuration's result
+  float4 *snap_pos = nullptr, *snap_vel = nullptr;
+  HIP_TRY(dalloc(&snap_pos, std::max<size_t>(n, 1)), "hipMalloc(tuner snapshot)");
+  if (dalloc(&snap_vel, std::max<size_t>(n, 1)) != hipSuccess) {
+    dfree(snap_pos);
+    return fail(FLUID_HIP_EHIP, "hipMalloc(tuner snapshot)");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const float t0 = c.time;
+  std::vector<float> ref(2 * soa), out(2 * soa);
+  int timed = 0, rejected = 0;
+  double base_ms = 0.0, best_ms = 0.0;
+
+  auto apply = [&](const Knobs& k) {
+    c.block = k.block;
+    c.xcd_remap = k.remap;
+    c.wave_mode = k.wave;
+    c.brick_shift = k.bshift;
+    compute_consts(c);
+    drop_graph(c);
+  };
+  // One trial: restore, run `trial_steps` substeps between two events, read
+  // the final state back in original order.  A trial that had to grow a
+  // table is run again, so that every timing is of plain substeps.
+  auto trial = [&](const Knobs& k, double& ms, std::vector<float>& res) -> int {
+    apply(k);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      int r;
+      if ((r = load_state(c, snap_pos, snap_vel))) return r;
+      c.time = t0;
+      const long long grows = c.grow_events;
+      HIP_TRY(hipEventRecord(e0, c.stream), "hipEventRecord");
+      if ((r = fluid_hip_step(ctx, trial_steps, 0))) return r;
+      HIP_TRY(hipEventRecord(e1, c.stream), "hipEventRecord");
+      HIP_TRY(hipEventSynchronize(e1), "hipEventSynchronize");
+      float f = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+      ms = f / trial_steps;
+      if (c.grow_events == grows) break;
+    }
+    if (n > 0) {
+      HIP_TRY(fluidhip::enqueue_download(c, c.stage, c.stage + soa), "download kernel");
+      HIP_TRY(hipMemcpyAsync(res.data(), c.stage, sizeof(float) * 2 * soa,
+                             hipMemcpyDeviceToHost, c.stream),
+              "read trial state");
+    }
+    HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+    return FLUID_HIP_OK;
+  };
+
+  auto run = [&]() -> int {
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    int r;
+    if ((r = save_state(c, snap_pos, snap_vel))) return r;
+    double ms = 0.0;
+    if ((r = trial(start, ms, ref))) return r;  // sizes the tables
+    if ((r = trial(start, base_ms, ref))) return r;
+    best_ms = base_ms;
+    Knobs cur = start;
+    const int blocks[] = {64, 128, 256, 512, 1024};
+    const int waves[] = {2, 3, 1};
+    const int shifts[] = {0, 1, 2, 3};
+    const int remaps[] = {1, 0};
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int knob = 0; knob < 4; ++knob) {
+        const int* vals = knob == 0 ? waves : knob == 1 ? blocks : knob == 2 ? shifts : remaps;
+        const int nv = knob == 0 ? 3 : knob == 1 ? 5 : knob == 2 ? 4 : 2;
+        for (int i = 0; i < nv; ++i) {
+          Knobs k = cur;
+          int& slot = knob == 0 ? k.wave : knob == 1 ? k.block : knob == 2 ? k.bshift : k.remap;
+          if (slot == vals[i]) continue;
+          slot = vals[i];
+          if ((r = trial(k, ms, out))) return r;
+          ++timed;
+          if (std::memcmp(out.data(), ref.data(), sizeof(float) * out.size()) != 0) {
+            ++rejected;  // not result-preserving: never chosen
+            continue;
+          }
+          // a switch must win by 3%, so timing noise does not flip knobs
+          if (ms < 0.97 * best_ms) {
+            best_ms = ms;
+            cur = k;
+          }
+        }
+      }
+    }
+    apply(cur);
+    if ((r = load_state(c, snap_pos, snap_vel))) return r;
+    c.time = t0;
+    HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+    if (best) {
+      best->block_size = cur.block;
+      best->xcd_remap = cur.remap;
+      best->wave_mode = cur.wave;
+      best->brick_edge = 1 << cur.bshift;
+      best->cell_capacity = c.cell_cap;
+      best->neighbor_capacity = c.nbr_cap;
+      best->device = c.device;
+      best->record_diag = c.record_diag ? 1 : 0;
+    }
+    return FLUID_HIP_OK;
+  };
+  rc = run();
+  if (rc) {
+    // leave the starting configuration and state behind
+    apply(start);
+    load_state(c, snap_pos, snap_vel);
+    hipStreamSynchronize(c.stream);
+    c.time = t0;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  dfree(snap_pos);
+  dfree(snap_vel);
+  if (rc) return rc;
+  if (report) {
+    report[0] = base_ms;
+    report[1] = best_ms;
+    report[2] = timed;
+    report[3] = rejected;
+  }
   return FLUID_HIP_OK;
 }
 

@@ -44,6 +44,7 @@ struct Ctx {
   int scan_blocks = 512;
   bool record_diag = false;
   int wave_mode = 2;
+  int brick_shift = 2;  // log2 of the cells per brick edge of the cell table
 
   // state (A = committed state between substeps)
   float4* posA = nullptr;  // xyz, w = original id bits

@@ -12,8 +12,6 @@
 namespace fluidhip {
 
 constexpr int kMaxPlanes = 8;
-constexpr int kBrick = 4;        // cells per brick edge
-constexpr int kBrickCells = 64;  // kBrick^3
 constexpr int kCoordLimit = 1 << 20;
 
 // Device error flags (sticky until the host clears them).
@@ -70,7 +68,8 @@ struct GridDesc {
   int ox, oy, oz;     // cell coordinate of table origin (bbox min - 1)
   int nbx, nby, nbz;  // bricks per axis
   int cell_off;       // first key of this scene
-  int ncells;         // nbx*nby*nbz*64
+  int ncells;         // nbx*nby*nbz bricks of 2^(3 bshift) cells
+  int bshift;         // log2 of the cells per brick edge (0..3)
 };
 
 __device__ __forceinline__ float poly6(float r2, const StepConsts& c) {
@@ -100,16 +99,26 @@ __device__ __forceinline__ float pow_ratio(float ratio, int n) {
   return powf(ratio, static_cast<float>(n));
 }
 
-__device__ __forceinline__ int morton_in_brick(int x, int y, int z) {
-  // 2-bit x, y, z interleaved: x0 y0 z0 x1 y1 z1
-  return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) | ((y & 2) << 3) |
-         ((z & 2) << 4);
+// Z-order of a cell inside its brick: the low `bits` bits of x, y, z
+// interleaved as x0 y0 z0 x1 y1 z1 ...
+__device__ __forceinline__ int morton_in_brick(int x, int y, int z, int bits) {
+  int m = 0;
+  for (int b = 0; b < bits; ++b)
+    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) |
+         (((z >> b) & 1) << (3 * b + 2));
+  return m;
 }
 
-// Key of the cell at table-relative coordinates (rx, ry, rz) >= 0.
+// Key of the cell at table-relative coordinates (rx, ry, rz) >= 0: bricks
+// in x-fastest order, cells in Z-order inside a brick.  The key only sets
+// where a cell's particles sit in the sorted arrays; every sum visits cells
+// in (dz, dy, dx) order and particles by id, so the brick edge never
+// changes a result.
 __device__ __forceinline__ int cell_key(int rx, int ry, int rz, const GridDesc& g) {
-  const int b = ((rz >> 2) * g.nby + (ry >> 2)) * g.nbx + (rx >> 2);
-  return g.cell_off + b * kBrickCells + morton_in_brick(rx & 3, ry & 3, rz & 3);
+  const int s = g.bshift;
+  const int m = (1 << s) - 1;
+  const int b = ((rz >> s) * g.nby + (ry >> s)) * g.nbx + (rx >> s);
+  return g.cell_off + (b << (3 * s)) + morton_in_brick(rx & m, ry & m, rz & m, s);
 }
 
 __device__ __forceinline__ int floor_cell(float v, float inv_h, bool& bad) {

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
                                                     int tile_size, int* __restrict__ bbox,
                                                     GridDesc* __restrict__ grid,
                                                     int* __restrict__ total_cells,
-                                                    long long cell_cap,
+                                                    long long cell_cap, int bshift,
                                                     unsigned* __restrict__ flags,
                                                     unsigned long long* __restrict__ nbr_total,
                                                     unsigned long long* __restrict__ need_cells,
@@ -287,8 +287,9 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
       const long long hx = bb[3], hy = bb[4], hz = bb[5];
       if (lx <= hx && ly <= hy && lz <= hz) {
         const long long sx = hx - lx + 3, sy = hy - ly + 3, sz = hz - lz + 3;
-        const long long bx = (sx + 3) / 4, by = (sy + 3) / 4, bz = (sz + 3) / 4;
-        nc = bx * by * bz * kBrickCells;
+        const long long e = 1ll << bshift;
+        const long long bx = (sx + e - 1) / e, by = (sy + e - 1) / e, bz = (sz + e - 1) / e;
+        nc = (bx * by * bz) << (3 * bshift);
         g.ox = static_cast<int>(lx - 1);
         g.oy = static_cast<int>(ly - 1);
         g.oz = static_cast<int>(lz - 1);
@@ -296,6 +297,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
         g.nby = static_cast<int>(by);
         g.nbz = static_cast<int>(bz);
       }
+      g.bshift = bshift;
       bb[0] = INT_MAX, bb[1] = INT_MAX, bb[2] = INT_MAX;
       bb[3] = INT_MIN, bb[4] = INT_MIN, bb[5] = INT_MIN;
     }
@@ -768,8 +770,9 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   const int rz = floor_cell(p.z, c.inv_h, bad) - g.oz;
   // The table has a one-cell margin around the bounding box of these same
   // cells; anything else would index outside it, so refuse instead.
-  if (active && (bad || rx < 1 || ry < 1 || rz < 1 || rx > 4 * g.nbx - 2 || ry > 4 * g.nby - 2 ||
-                 rz > 4 * g.nbz - 2)) {
+  const int ext = 1 << g.bshift;  // cells per brick edge
+  if (active && (bad || rx < 1 || ry < 1 || rz < 1 || rx > ext * g.nbx - 2 ||
+                 ry > ext * g.nby - 2 || rz > ext * g.nbz - 2)) {
     atomicOr(flags, kFlagRange);
     active = false;
   }
@@ -1448,7 +1451,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   }));
   FH_TRY(L("grid_setup", [&] {
     k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.seg, c.tile_bbox, bs, c.bbox, c.grid,
-                                    c.total_cells, c.cell_cap,
+                                    c.total_cells, c.cell_cap, c.brick_shift,
                                     c.flags, c.nbr_total, c.need_cells, c.steps_started);
   }));
   FH_TRY(L("cell_count", [&] {

@@ -42,7 +42,7 @@ class FluidHipConfig(C.Structure):
     _fields_ = [
         ("cell_capacity", C.c_int64), ("neighbor_capacity", C.c_int64), ("device", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("record_diag", C.c_int32),
-        ("wave_mode", C.c_int32),
+        ("wave_mode", C.c_int32), ("brick_edge", C.c_int32),
     ]
 
 
@@ -80,8 +80,10 @@ _SIGS = {
                                          C.POINTER(C.c_int64)]),
 }
 
-# Differentiable rollout.
+# Autotuning and the differentiable rollout.
 _OPTIONAL_SIGS: dict = {
+    "fluid_hip_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(FluidHipConfig),
+                                     C.POINTER(C.c_double)]),
     "fluid_hip_forward_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fluid_hip_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
@@ -161,7 +163,7 @@ class HipSolver:
     counts: particle count per scene (defaults to [state.n] via upload)."""
 
     def __init__(self, params, counts, *, cell_capacity=0, neighbor_capacity=0, block_size=256,
-                 xcd_remap=True, record_diag=False, device=0, wave_mode=0):
+                 xcd_remap=True, record_diag=False, device=0, wave_mode=0, brick_edge=0):
         self.lib = load_library()
         plist = list(params) if isinstance(params, (list, tuple)) else [params]
         counts = [int(c) for c in counts]
@@ -177,7 +179,7 @@ class HipSolver:
         cnt = (C.c_int64 * self.n_scenes)(*counts)
         cfg = FluidHipConfig(int(cell_capacity), int(neighbor_capacity), int(device),
                              int(block_size), int(bool(xcd_remap)), int(bool(record_diag)),
-                             int(wave_mode))
+                             int(wave_mode), int(brick_edge))
         h = C.c_void_p()
         _check(self.lib.fluid_hip_create(C.byref(h), self.n_scenes, cnt, arr, C.byref(cfg)))
         self.h = h
@@ -275,6 +277,18 @@ class HipSolver:
         keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events",
                 "record_bytes", "recomputed")
         return dict(zip(keys, [int(x) for x in out]))
+
+    def autotune(self, trial_steps=2):
+        """Time the result-preserving launch variants on the current state
+        and keep the fastest (the state is restored).  Returns the chosen
+        knobs and the timings."""
+        cfg = FluidHipConfig()
+        rep = (C.c_double * 4)()
+        _check(self.lib.fluid_hip_autotune(self.h, int(trial_steps), C.byref(cfg), rep))
+        return {"block_size": cfg.block_size, "xcd_remap": cfg.xcd_remap,
+                "wave_mode": cfg.wave_mode, "brick_edge": cfg.brick_edge,
+                "start_ms_per_step": rep[0], "best_ms_per_step": rep[1],
+                "candidates_timed": int(rep[2]), "candidates_rejected": int(rep[3])}
 
     def forward_record(self, nsteps, checkpoint_every=10):
         """Run `nsteps` substeps keeping what the reverse pass needs (state

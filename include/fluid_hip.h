@@ -84,6 +84,8 @@ typedef struct fluid_hip_config {
   int32_t wave_mode;          /* gathers of wavefronts that share one cell: 0 default (2),
                                  1 per-lane lists only, 2 shared candidate tiles,
                                  3 shared tiles without the coincident-tile shortcut */
+  int32_t brick_edge;         /* cells per brick edge of the cell table's Z-order:
+                                 0 default (4), 1, 2, 4 or 8 */
 } fluid_hip_config;
 
 typedef struct fluid_hip_ctx fluid_hip_ctx;
@@ -175,6 +177,22 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
  * (checkpoints and tapes), out[7] substeps the last backward pass
  * recomputed.  `n` entries are written (<= 8). */
 int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
+
+/* ---- autotuning ---------------------------------------------------------- */
+
+/* No counterpart in the reference.  Picks the launch configuration of this
+ * context by timing: the gather mode of wavefronts sharing a cell (wave_mode
+ * 1..3), the workgroup size (64..1024), the cell table's brick edge (1, 2, 4,
+ * 8) and the XCD-aware tile order, tuned one knob at a time over two passes.
+ * Every candidate runs `trial_steps` substeps from the current state between
+ * two events; its final state must equal the starting configuration's bit
+ * for bit, or it is rejected.  The fastest candidate (by at least 3%) is
+ * kept; the state and time are restored.  `best` receives the chosen
+ * configuration, report[0..3] the starting and chosen ms per substep, the
+ * candidates timed and the candidates rejected (either may be NULL).  A
+ * recorded rollout is discarded. */
+int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, fluid_hip_config* best,
+                       double* report);
 
 /* ---- differentiable rollout ---------------------------------------------- */
 

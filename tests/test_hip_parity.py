@@ -37,24 +37,28 @@ def run_hip(p, pos, vel, steps, record_diag=True, graph=False, **kw):
     return out
 
 
-def brick_keys(cells, desc):
-    """Sort key of each cell: scene offset + brick index * 64 + Z-order of
-    the cell inside its 4x4x4 brick (same definition as the kernels)."""
+def brick_keys(cells, desc, edge=4):
+    """Sort key of each cell: scene offset + brick index * edge^3 + Z-order
+    of the cell inside its brick (bits of x, y, z interleaved; the same
+    definition as the kernels)."""
     ox, oy, oz, nbx, nby, nbz, off, _ = [int(v) for v in desc]
     rx, ry, rz = cells[:, 0] - ox, cells[:, 1] - oy, cells[:, 2] - oz
-    b = ((rz >> 2) * nby + (ry >> 2)) * nbx + (rx >> 2)
-    m = (rx & 1) | ((ry & 1) << 1) | ((rz & 1) << 2) | ((rx & 2) << 2) | ((ry & 2) << 3) | \
-        ((rz & 2) << 4)
-    return off + b * 64 + m
+    bits = edge.bit_length() - 1
+    b = ((rz >> bits) * nby + (ry >> bits)) * nbx + (rx >> bits)
+    m = np.zeros_like(rx)
+    for k in range(bits):
+        m |= (((rx >> k) & 1) << (3 * k)) | (((ry >> k) & 1) << (3 * k + 1)) | \
+             (((rz >> k) & 1) << (3 * k + 2))
+    return off + b * edge ** 3 + m
 
 
-def expected_desc(cells):
+def expected_desc(cells, edge=4):
     lo = cells.min(axis=0)
     hi = cells.max(axis=0)
     size = hi - lo + 3
-    nb = (size + 3) // 4
+    nb = (size + edge - 1) // edge
     return np.array([lo[0] - 1, lo[1] - 1, lo[2] - 1, nb[0], nb[1], nb[2], 0,
-                     int(nb.prod()) * 64], np.int64)
+                     int(nb.prod()) * edge ** 3], np.int64)
 
 
 @pytest.mark.parametrize("name", [n for n in golden_names() if not n.startswith("bruteforce")])
@@ -143,6 +147,47 @@ def test_launch_shapes_do_not_change_results(block, remap):
     a = run_hip(sc.params, sc.state.pos, sc.state.vel, 3)
     b = run_hip(sc.params, sc.state.pos, sc.state.vel, 3, block_size=block, xcd_remap=remap)
     assert same_bits(a["pos"], b["pos"]) and same_bits(a["vel"], b["vel"])
+
+
+@pytest.mark.parametrize("edge", [1, 2, 8])
+def test_brick_edge_orders_cells_but_keeps_results(edge):
+    sc = scenes.dam_break(seed=7, dims=(20, 16, 12), flags=True)
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=1, diag=True)
+    r = run_hip(sc.params, sc.state.pos, sc.state.vel, 1, brick_edge=edge)
+    desc = expected_desc(o["cell"], edge)
+    assert np.array_equal(r["desc"][0], desc)
+    keys = brick_keys(o["cell"], desc, edge)
+    order = np.lexsort((np.arange(sc.state.n), keys))
+    assert np.array_equal(r["sorted_ids"], order)
+    assert np.array_equal(r["keys"], keys[order])
+    assert same_bits(r["pos"], o["pos"]) and same_bits(r["vel"], o["vel"])
+    a = run_hip(sc.params, sc.state.pos, sc.state.vel, 5)
+    b = run_hip(sc.params, sc.state.pos, sc.state.vel, 5, brick_edge=edge)
+    assert same_bits(a["pos"], b["pos"]) and same_bits(a["vel"], b["vel"])
+
+
+def test_autotune_restores_state_and_keeps_results():
+    sc = scenes.dam_break(seed=13, dims=(24, 20, 16), flags=True)
+    n = sc.state.n
+    with hip.HipSolver(sc.params, [n]) as s:
+        s.upload(sc.state.pos, sc.state.vel, 0.0)
+        s.step(5)
+        before = s.download()
+        rep = s.autotune(trial_steps=2)
+        after = s.download()
+        assert same_bits(before.pos, after.pos) and same_bits(before.vel, after.vel)
+        assert after.time == before.time
+        assert rep["block_size"] in (64, 128, 256, 512, 1024)
+        assert rep["wave_mode"] in (1, 2, 3) and rep["brick_edge"] in (1, 2, 4, 8)
+        assert rep["candidates_rejected"] == 0 and rep["candidates_timed"] >= 12
+        assert 0 < rep["best_ms_per_step"] <= rep["start_ms_per_step"]
+        s.step(7)
+        tuned = s.download()
+    with hip.HipSolver(sc.params, [n]) as s:
+        s.upload(sc.state.pos, sc.state.vel, 0.0)
+        s.step(12)
+        plain = s.download()
+    assert same_bits(tuned.pos, plain.pos) and same_bits(tuned.vel, plain.vel)
 
 
 def test_capacity_auto_growth_keeps_results():
