@@ -339,6 +339,9 @@ def main():
                     help="ensemble: batched + hipGraph against a per-scene loop; adjoint: "
                          "gradient of a rollout against the rollout (each its own JSON line)")
     ap.add_argument("--every", type=int, default=10, help="adjoint checkpoint interval")
+    ap.add_argument("--autotune", type=int, default=2, metavar="TRIAL_STEPS",
+                    help="autotune the launch configuration on the warmed-up state before "
+                         "the timed region (0: off)")
     ap.add_argument("--scenes", type=int, default=256, help="ensemble size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (counter calibration)")
@@ -375,7 +378,9 @@ def main():
     s.upload(sc.state.pos, sc.state.vel, 0.0)
     s.step(args.warmup, graph=args.graph)
     start_state = s.download() if not args.no_cpu else None
-    s.step(1, graph=args.graph)  # settles capacity growth after the download
+    # the tuner's trials start from this state and leave it in place (they
+    # also size the tables for it); the timed region starts here too
+    tuned = s.autotune(args.autotune) if args.autotune > 0 else None
     s.sync()
     prof = not args.no_profile and not args.graph
     if prof:
@@ -435,8 +440,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice)",
         "config": {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]} all flags, 4 iters",
-                   "particles": n, "graph": bool(args.graph), "block": args.block,
-                   "xcd_remap": bool(args.xcd_remap)},
+                   "particles": n, "graph": bool(args.graph),
+                   "block": tuned["block_size"] if tuned else args.block,
+                   "xcd_remap": bool(tuned["xcd_remap"] if tuned else args.xcd_remap),
+                   "wave_mode": tuned["wave_mode"] if tuned else 2,
+                   "brick_edge": tuned["brick_edge"] if tuned else 4},
+        "autotune": tuned,
         "particle_steps_per_s": steps_per_s * n,
         "step_hbm_fraction": step_gbs / HBM_PEAK_GBS,
         "roofline": roofline,
