@@ -955,13 +955,14 @@ int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total
   return FLUID_HIP_OK;
 }
 
-int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, fluid_hip_config* best,
-                       double* report) {
+int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
+                       fluid_hip_config* best, double* report) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   Ctx& c = ctx->c;
   if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
   if (trial_steps < 1) return fail(FLUID_HIP_EINVAL, "trial_steps < 1");
+  if (knobs == 0) knobs = FLUID_HIP_TUNE_DEFAULT;
   struct Knobs {
     int block, remap, wave, bshift;
   };
@@ -1035,8 +1036,11 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, fluid_hip_config
     const int waves[] = {2, 3, 1};
     const int shifts[] = {0, 1, 2, 3};
     const int remaps[] = {1, 0};
+    const int knob_bit[4] = {FLUID_HIP_TUNE_WAVE_MODE, FLUID_HIP_TUNE_BLOCK,
+                             FLUID_HIP_TUNE_BRICK, FLUID_HIP_TUNE_XCD};
     for (int pass = 0; pass < 2; ++pass) {
       for (int knob = 0; knob < 4; ++knob) {
+        if (!(knobs & knob_bit[knob])) continue;
         const int* vals = knob == 0 ? waves : knob == 1 ? blocks : knob == 2 ? shifts : remaps;
         const int nv = knob == 0 ? 3 : knob == 1 ? 5 : knob == 2 ? 4 : 2;
         for (int i = 0; i < nv; ++i) {

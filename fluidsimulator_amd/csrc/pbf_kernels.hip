@@ -29,6 +29,7 @@
 
 #include <climits>
 
+#include "exact_sum.h"
 #include "pbf_ctx.h"
 #include "pbf_device.h"
 
@@ -859,9 +860,53 @@ struct ListCursor {
   }
 };
 
+// The pair arithmetic of one gather pass as five operations:
+//   same(leader)  this lane's pair state (what its terms depend on besides
+//                 the candidate) equals the leader lane's, bit for bit
+//   bind(leader)  take the leader's state as the group's
+//   gterm(q)      the term of candidate q against the group's state
+//   add(t)        accumulate one term (sums run in list order)
+//   pair(q)       add(the term of q against this lane's own state)
+// same and bind broadcast, so they run on the whole wavefront.
+template <typename Same, typename Bind, typename GTerm, typename Add, typename Pair>
+struct PairOps {
+  Same same;
+  Bind bind;
+  GTerm gterm;
+  Add add;
+  Pair pair;
+};
+template <typename Same, typename Bind, typename GTerm, typename Add, typename Pair>
+__device__ __forceinline__ PairOps<Same, Bind, GTerm, Add, Pair> pair_ops(Same s, Bind b, GTerm g,
+                                                                          Add a, Pair p) {
+  return {s, b, g, a, p};
+}
+
+__device__ __forceinline__ bool same_bits(const float3& a, const float3& b) {
+  return __float_as_int(a.x) == __float_as_int(b.x) && __float_as_int(a.y) == __float_as_int(b.y) &&
+         __float_as_int(a.z) == __float_as_int(b.z);
+}
+__device__ __forceinline__ bool same_bits(const float4& a, const float4& b) {
+  return same_bits(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z)) &&
+         __float_as_int(a.w) == __float_as_int(b.w);
+}
+__device__ __forceinline__ bool same_bits(float a, float b) {
+  return __float_as_int(a) == __float_as_int(b);
+}
+
+// Lanes of one group that share their pair state bit for bit form a class;
+// a group with at most this many classes walks its tiles once per class
+// with terms computed once per candidate, otherwise once with every lane
+// evaluating every candidate.  Walks of one wavefront run one after the
+// other, and holding several classes' terms at once costs registers (fewer
+// resident wavefronts): on the piled-up million-particle dam break one
+// class is fastest (4.26 substeps/s; 4 class walks 3.38; 4 classes in one
+// walk 3.64; no class walks 3.99).
+constexpr int kMaxClasses = 1;
+
 // Gathers over the neighbors of every lane of a wavefront, in each lane's
 // list order:
-//  * lanes in dense cells: one cooperative walk per distinct dense cell of
+//  * lanes in dense cells: cooperative walks of each distinct dense cell of
 //    the wavefront; candidate tiles are loaded once (coalesced) and handed
 //    lane to lane with readlane; each lane takes exactly the candidates its
 //    own list holds;
@@ -870,47 +915,91 @@ struct ListCursor {
 //    neighbors in the tile contribute: 0 one by one through pair_u,
 //    1 nothing at all (exact signed zeros), 2 `count` identical terms
 //    (on_count); flag is tflag(tile, m), evaluated by the whole wavefront;
+//  * lanes that share their pair state bit for bit (particles piled up at
+//    one point) get the same term from each candidate: in their walk, lane
+//    j computes candidate j's term once and every lane adds the broadcast
+//    terms of its own neighbors in order;
 //  * other lanes walk their own run-length-encoded lists.
-template <typename Load, typename Pair, typename TFlag, typename Prep, typename PairU,
+// Every lane belongs to exactly one walk, so each sum still runs over the
+// lane's list in order.
+template <typename Load, typename Ops, typename TFlag, typename Prep, typename PairU,
           typename OnCount>
 __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, const CellView& cv,
-                                       const LaneCell& L, Load&& load, Pair&& pair,
+                                       const LaneCell& L, Load&& load, Ops&& ops,
                                        TFlag&& tflag, Prep&& prep, PairU&& pair_u,
                                        OnCount&& on_count) {
+  // one walk of the group `gr` by the lanes with `in` set
+  auto walk = [&](const Group& gr, bool in, bool uni) {
+    ListCursor lc{nv.list + nv.off[L.s], nv.nw[L.s], 0, INT_MAX, INT_MAX};
+    if (in) lc.advance();
+    group_tiles(gr, cv.start, load, [&](int base, int m, const auto& tile) {
+      float3 cp;
+      if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
+        const bool tf = tflag(tile, m);
+        const int mode = in ? prep(cp, tf) : 1;
+        if (__all(mode != 0)) {
+          const int cnt = lc.take(base, m);
+          if (mode == 2) on_count(cnt);
+          return;
+        }
+        for (int j = 0; j < m; ++j) {
+          const auto q = bcast(tile, j);
+          if (lc.hit(base + j)) {
+            if (mode == 0)
+              pair_u(q);
+            else if (mode == 2)
+              on_count(1);
+          }
+        }
+        return;
+      }
+      if (uni) {
+        const auto t = ops.gterm(tile);
+        for (int j = 0; j < m; ++j) {
+          const auto tj = bcast(t, j);
+          if (lc.hit(base + j)) ops.add(tj);
+        }
+        return;
+      }
+      for (int j = 0; j < m; ++j) {
+        const auto q = bcast(tile, j);
+        if (lc.hit(base + j)) ops.pair(q);
+      }
+    });
+  };
   if (c.coop)
     for_each_key_group(L.dense, L.key, [&](int leader, bool mine) {
-      ListCursor lc{nv.list + nv.off[L.s], nv.nw[L.s], 0, INT_MAX, INT_MAX};
-      if (mine) lc.advance();
-      group_tiles(group_of(c, cv, L, leader), cv.start, load,
-                  [&](int base, int m, const auto& tile) {
-                    float3 cp;
-                    if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
-                      const bool tf = tflag(tile, m);
-                      const int mode = mine ? prep(cp, tf) : 1;
-                      if (__all(mode != 0)) {
-                        const int cnt = lc.take(base, m);
-                        if (mode == 2) on_count(cnt);
-                        return;
-                      }
-                      for (int j = 0; j < m; ++j) {
-                        const auto q = bcast(tile, j);
-                        if (lc.hit(base + j)) {
-                          if (mode == 0)
-                            pair_u(q);
-                          else if (mode == 2)
-                            on_count(1);
-                        }
-                      }
-                      return;
-                    }
-                    for (int j = 0; j < m; ++j) {
-                      const auto q = bcast(tile, j);
-                      if (lc.hit(base + j)) pair(q);
-                    }
-                  });
+      const Group gr = group_of(c, cv, L, leader);
+      // count the group's state classes (up to one past the limit)
+      int classes = 0;
+      if (c.coop_bulk) {
+        bool pend = mine;
+        while (classes <= kMaxClasses) {
+          const unsigned long long pm = __ballot(pend);
+          if (pm == 0ull) break;
+          const bool eq = ops.same(__builtin_ctzll(pm));
+          pend = pend && !eq;
+          ++classes;
+        }
+      }
+      if (classes == 0 || classes > kMaxClasses) {
+        walk(gr, mine, false);
+        return;
+      }
+      bool pend = mine;
+      while (true) {
+        const unsigned long long pm = __ballot(pend);
+        if (pm == 0ull) break;
+        const int cl = __builtin_ctzll(pm);
+        const bool eq = ops.same(cl);  // broadcasts: whole wavefront
+        const bool in = pend && eq;
+        ops.bind(cl);
+        walk(gr, in, true);
+        pend = pend && !in;
+      }
     });
   if (L.active && !L.dense)
-    for_each_neighbor(nv, L.s, load, [&](int, const auto& q) { pair(q); });
+    for_each_neighbor(nv, L.s, load, [&](int, const auto& q) { ops.pair(q); });
 }
 
 // Spiky factor of a pair at squared distance r2 < h2: pairs closer than
@@ -958,6 +1047,27 @@ __device__ __forceinline__ EPosW bcast(const EPosW& e, int j) {
 
 __device__ __forceinline__ bool tile_all(bool ok) { return __ballot(ok) == ~0ull; }
 
+// Per-candidate terms of the passes (what one neighbor adds to the sums).
+struct LTerm {  // lambda: rho, gradient sum, squared gradient
+  float w, gx, gy, gz, j2;
+  int grad;
+};
+struct V3Term {  // delta, XSPH, omega, eta: a 3-vector, or nothing
+  float x, y, z;
+  int on;
+};
+__device__ __forceinline__ LTerm bcast(const LTerm& t, int j) {
+  return LTerm{bcast(t.w, j), bcast(t.gx, j), bcast(t.gy, j), bcast(t.gz, j), bcast(t.j2, j),
+               bcast(t.grad, j)};
+}
+__device__ __forceinline__ V3Term bcast(const V3Term& t, int j) {
+  return V3Term{bcast(t.x, j), bcast(t.y, j), bcast(t.z, j), bcast(t.on, j)};
+}
+__device__ __forceinline__ bool same_term(const LTerm& a, const LTerm& b) {
+  return a.grad == b.grad && same_bits(a.w, b.w) && same_bits(a.gx, b.gx) &&
+         same_bits(a.gy, b.gy) && same_bits(a.gz, b.gz) && same_bits(a.j2, b.j2);
+}
+
 // ---------------------------------------------------------------- solver
 //
 // Coincident pairs (dx = dy = dz = 0) are common where particles are pushed
@@ -984,65 +1094,68 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
   const float3 p = ld3(P, s);
   float rho = 0.0f, gsx = 0.0f, gsy = 0.0f, gsz = 0.0f, sg2 = 0.0f;
   const float ngs = -c.grad_scale;
-  // per-pair increments of a uniform tile
-  float iw = 0.0f, igx = 0.0f, igy = 0.0f, igz = 0.0f, ij2 = 0.0f;
-  bool igrad = false;
-  auto pair = [&](const EPos& q) {
-    const float dx = p.x - q.p.x;
-    const float dy = p.y - q.p.y;
-    const float dz = p.z - q.p.z;
-    if (dx == 0.0f && dy == 0.0f && dz == 0.0f) {
-      rho += c.w0;  // poly6(0); f * (+-0) terms leave the sums unchanged
-      return;
-    }
+  // what a candidate at q adds for a particle at ps
+  auto term = [&](float3 ps, float3 q) -> LTerm {
+    const float dx = ps.x - q.x;
+    const float dy = ps.y - q.y;
+    const float dz = ps.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    rho += poly6(r2, c);
-    if (r2 < c.h2) {
-      const float f = pair_spiky(r2, c);
-      const float gx = f * dx, gy = f * dy, gz = f * dz;
-      gsx += gx;
-      gsy += gy;
-      gsz += gz;
-      const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
-      sg2 += jx * jx + jy * jy + jz * jz;
+    // coincident: poly6(0), and the f * (+-0) terms leave the sums unchanged
+    const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+    const bool grad = !same && r2 < c.h2;
+    const float f = grad ? pair_spiky(r2, c) : 0.0f;
+    const float gx = f * dx, gy = f * dy, gz = f * dz;
+    const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
+    return LTerm{same ? c.w0 : poly6(r2, c), gx, gy, gz, jx * jx + jy * jy + jz * jz,
+                 grad ? 1 : 0};
+  };
+  // Identical terms of consecutive neighbors are counted and added in one
+  // go (exact_sum.h) before anything else is added.
+  LTerm run{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0};
+  int run_n = 0;
+  auto flush = [&]() {
+    if (run_n == 0) return;
+    rho = add_repeat(rho, run.w, run_n);
+    if (run.grad) {
+      gsx = add_repeat(gsx, run.gx, run_n);
+      gsy = add_repeat(gsy, run.gy, run_n);
+      gsz = add_repeat(gsz, run.gz, run_n);
+      sg2 = add_repeat(sg2, run.j2, run_n);
+    }
+    run_n = 0;
+  };
+  auto add = [&](LTerm t) {
+    flush();
+    rho += t.w;
+    if (t.grad) {
+      gsx += t.gx;
+      gsy += t.gy;
+      gsz += t.gz;
+      sg2 += t.j2;
     }
   };
+  float3 gp = p;      // the group's position
+  LTerm tile_t = run;  // the term of the current one-point tile
   gather(
-      c, nv, cv, L, [&](int t) { return EPos{ld3(P, t)}; }, pair,
+      c, nv, cv, L, [&](int t) { return EPos{ld3(P, t)}; },
+      pair_ops([&](int leader) { return same_bits(bcast(p, leader), p); },
+               [&](int leader) { gp = bcast(p, leader); },
+               [&](const EPos& q) { return term(gp, q.p); }, add,
+               [&](const EPos& q) { add(term(p, q.p)); }),
       [&](const EPos&, int) { return true; },
       [&](const float3& cp, bool) {
-        const float dx = p.x - cp.x;
-        const float dy = p.y - cp.y;
-        const float dz = p.z - cp.z;
-        igrad = false;
-        if (dx == 0.0f && dy == 0.0f && dz == 0.0f) {
-          iw = c.w0;
-          return 2;
-        }
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        iw = poly6(r2, c);
-        if (r2 < c.h2) {
-          const float f = pair_spiky(r2, c);
-          igx = f * dx, igy = f * dy, igz = f * dz;
-          const float jx = ngs * igx, jy = ngs * igy, jz = ngs * igz;
-          ij2 = jx * jx + jy * jy + jz * jz;
-          igrad = true;
-        }
+        tile_t = term(p, cp);
         return 2;
       },
       [&](const EPos&) {},
       [&](int cnt) {
         // cnt identical neighbors: each accumulator receives the same term
         // cnt times, in order
-        for (int k = 0; k < cnt; ++k) rho += iw;
-        if (igrad)
-          for (int k = 0; k < cnt; ++k) {
-            gsx += igx;
-            gsy += igy;
-            gsz += igz;
-            sg2 += ij2;
-          }
+        if (run_n > 0 && !same_term(run, tile_t)) flush();
+        run = tile_t;
+        run_n += cnt;
       });
+  flush();
   if (!L.active) return;
   rho += c.w0;
   rho *= c.mass;
@@ -1077,36 +1190,51 @@ __global__ __launch_bounds__(1024) void k_delta(
     // uniform-tile geometry
     float hdx = 0.f, hdy = 0.f, hdz = 0.f, hf = 0.f, hsc = 0.f;
     bool hsame = false;
-    auto add_term = [&](float dx, float dy, float dz, bool same, float f, float W, float lj) {
-      float sv = pi.w + lj;
+    // what a neighbor with lambda lj adds for a particle with lambda ps.w
+    auto dterm = [&](const float4& ps, float dx, float dy, float dz, bool same, float f, float W,
+                     float lj) {
+      V3Term t{0.0f, 0.0f, 0.0f, 0};
+      float sv = ps.w + lj;
       if (c.scorr_on) {
         const float ratio = W * c.scorr_inv_wdq;
         const float scorr = c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
         sv += scorr;
       }
       const float sf = sv * f;
-      if (same && isfinite(sf)) return;  // (sv f) * (+-0): +-0
-      ddx += sf * dx;
-      ddy += sf * dy;
-      ddz += sf * dz;
+      if (same && isfinite(sf)) return t;  // (sv f) * (+-0): +-0
+      t.x = sf * dx;
+      t.y = sf * dy;
+      t.z = sf * dz;
+      t.on = 1;
+      return t;
     };
+    auto term = [&](const float4& ps, const EPosL& q) {
+      const float dx = ps.x - q.p.x;
+      const float dy = ps.y - q.p.y;
+      const float dz = ps.z - q.p.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (!(r2 < c.h2)) return V3Term{0.0f, 0.0f, 0.0f, 0};
+      const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+      return dterm(ps, dx, dy, dz, same, same ? c.f_min : pair_spiky(r2, c),
+                   same ? c.w0 : poly6(r2, c), q.l);
+    };
+    auto add = [&](const V3Term& t) {
+      if (!t.on) return;
+      ddx += t.x;
+      ddy += t.y;
+      ddz += t.z;
+    };
+    float4 gpi = pi;  // the group's position and lambda
     gather(
         c, nv, cv, L,
         [&](int t) {
           const float4 q = P[t];
           return EPosL{make_float3(q.x, q.y, q.z), q.w};
         },
-        [&](const EPosL& q) {
-          const float dx = pi.x - q.p.x;
-          const float dy = pi.y - q.p.y;
-          const float dz = pi.z - q.p.z;
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < c.h2) {
-            const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-            add_term(dx, dy, dz, same, same ? c.f_min : pair_spiky(r2, c),
-                     same ? c.w0 : poly6(r2, c), q.l);
-          }
-        },
+        pair_ops([&](int leader) { return same_bits(bcast(pi, leader), pi); },
+                 [&](int leader) { gpi = bcast(pi, leader); },
+                 [&](const EPosL& q) { return term(gpi, q); }, add,
+                 [&](const EPosL& q) { add(term(pi, q)); }),
         // bounded lambdas keep (sv f) finite for coincident neighbors
         [&](const EPosL& tile, int m) {
           const int lane = threadIdx.x & (kWave - 1);
@@ -1124,7 +1252,7 @@ __global__ __launch_bounds__(1024) void k_delta(
           hsc = hsame ? c.w0 : poly6(r2, c);  // W, for s_corr
           return 0;
         },
-        [&](const EPosL& q) { add_term(hdx, hdy, hdz, hsame, hf, hsc, q.l); },
+        [&](const EPosL& q) { add(dterm(pi, hdx, hdy, hdz, hsame, hf, hsc, q.l)); },
         [&](int) {});
     if (!L.active) return;
     ddx *= c.inv_density;
@@ -1178,11 +1306,27 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float hW = 0.0f;
   float3 tv = make_float3(0.f, 0.f, 0.f);
-  auto add_term = [&](float W, const EPosV& q) {
-    ax += (q.v.x - vi.x) * W * q.irho;
-    ay += (q.v.y - vi.y) * W * q.irho;
-    az += (q.v.z - vi.z) * W * q.irho;
+  const float3 vi3 = make_float3(vi.x, vi.y, vi.z);
+  // what a neighbor adds for a particle moving at vs
+  auto xterm = [&](const float3& vs, float W, const EPosV& q) {
+    return V3Term{(q.v.x - vs.x) * W * q.irho, (q.v.y - vs.y) * W * q.irho,
+                  (q.v.z - vs.z) * W * q.irho, 1};
   };
+  auto term = [&](const float3& xs, const float3& vs, const EPosV& q) {
+    const float dx = xs.x - q.p.x;
+    const float dy = xs.y - q.p.y;
+    const float dz = xs.z - q.p.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < c.h2)) return V3Term{0.0f, 0.0f, 0.0f, 0};
+    return xterm(vs, poly6(r2, c), q);
+  };
+  auto add = [&](const V3Term& t) {
+    if (!t.on) return;
+    ax += t.x;
+    ay += t.y;
+    az += t.z;
+  };
+  float3 gx3 = xi, gv3 = vi3;  // the group's position and velocity
   gather(
       c, nv, cv, L,
       [&](int t) {
@@ -1190,13 +1334,16 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
         return EPosV{ld3(pos, t), make_float3(v.x, v.y, v.z),
                      (v.w > 0.0f) ? (c.mass / v.w) : 0.0f};
       },
-      [&](const EPosV& q) {
-        const float dx = xi.x - q.p.x;
-        const float dy = xi.y - q.p.y;
-        const float dz = xi.z - q.p.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < c.h2) add_term(poly6(r2, c), q);
-      },
+      pair_ops(
+          [&](int leader) {
+            return same_bits(bcast(xi, leader), xi) && same_bits(bcast(vi3, leader), vi3);
+          },
+          [&](int leader) {
+            gx3 = bcast(xi, leader);
+            gv3 = bcast(vi3, leader);
+          },
+          [&](const EPosV& q) { return term(gx3, gv3, q); }, add,
+          [&](const EPosV& q) { add(term(xi, vi3, q)); }),
       // one velocity in the whole tile (bitwise), finite m/rho everywhere
       [&](const EPosV& tile, int m) {
         const int lane = threadIdx.x & (kWave - 1);
@@ -1220,7 +1367,7 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
           return 1;
         return 0;
       },
-      [&](const EPosV& q) { add_term(hW, q); }, [&](int) {});
+      [&](const EPosV& q) { add(xterm(vi3, hW, q)); }, [&](int) {});
   if (!L.active) return;
   float vx = vi.x + c.visc_c * ax;
   float vy = vi.y + c.visc_c * ay;
@@ -1248,28 +1395,43 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   float hgx = 0.f, hgy = 0.f, hgz = 0.f;
   bool hsame = false;
-  auto add_term = [&](bool same, float gx, float gy, float gz, const float3& vj) {
-    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+  // what a neighbor moving at vj adds for a particle moving at vs
+  auto wterm = [&](const float3& vs, bool same, float gx, float gy, float gz, const float3& vj) {
+    const float dvx = vj.x - vs.x, dvy = vj.y - vs.y, dvz = vj.z - vs.z;
     if (same && isfinite(dvx) && isfinite(dvy) && isfinite(dvz))
-      return;  // dv x (f * (+-0)) is a signed zero
-    ox += dvy * gz - dvz * gy;
-    oy += dvz * gx - dvx * gz;
-    oz += dvx * gy - dvy * gx;
+      return V3Term{0.0f, 0.0f, 0.0f, 0};  // dv x (f * (+-0)) is a signed zero
+    return V3Term{dvy * gz - dvz * gy, dvz * gx - dvx * gz, dvx * gy - dvy * gx, 1};
   };
+  auto term = [&](const float3& xs, const float3& vs, const EPosV3& q) {
+    const float dx = xs.x - q.p.x;
+    const float dy = xs.y - q.p.y;
+    const float dz = xs.z - q.p.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < c.h2)) return V3Term{0.0f, 0.0f, 0.0f, 0};
+    const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+    const float f = pair_spiky(r2, c);
+    return wterm(vs, same, f * dx, f * dy, f * dz, q.v);
+  };
+  auto add = [&](const V3Term& t) {
+    if (!t.on) return;
+    ox += t.x;
+    oy += t.y;
+    oz += t.z;
+  };
+  float3 gx3 = xi, gv3 = vi;  // the group's position and velocity
   const bool own_ok = fabsf(vi.x) < kHalfMax && fabsf(vi.y) < kHalfMax && fabsf(vi.z) < kHalfMax;
   gather(
       c, nv, cv, L, [&](int t) { return EPosV3{ld3(pos, t), ld3(vel, t)}; },
-      [&](const EPosV3& q) {
-        const float dx = xi.x - q.p.x;
-        const float dy = xi.y - q.p.y;
-        const float dz = xi.z - q.p.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < c.h2) {
-          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-          const float f = pair_spiky(r2, c);
-          add_term(same, f * dx, f * dy, f * dz, q.v);
-        }
-      },
+      pair_ops(
+          [&](int leader) {
+            return same_bits(bcast(xi, leader), xi) && same_bits(bcast(vi, leader), vi);
+          },
+          [&](int leader) {
+            gx3 = bcast(xi, leader);
+            gv3 = bcast(vi, leader);
+          },
+          [&](const EPosV3& q) { return term(gx3, gv3, q); }, add,
+          [&](const EPosV3& q) { add(term(xi, vi, q)); }),
       // finite velocity differences make coincident terms signed zeros
       [&](const EPosV3& tile, int m) {
         const int lane = threadIdx.x & (kWave - 1);
@@ -1288,7 +1450,7 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
         hgx = f * dx, hgy = f * dy, hgz = f * dz;
         return 0;
       },
-      [&](const EPosV3& q) { add_term(hsame, hgx, hgy, hgz, q.v); }, [&](int) {});
+      [&](const EPosV3& q) { add(wterm(vi, hsame, hgx, hgy, hgz, q.v)); }, [&](int) {});
   if (!L.active) return;
   omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
 }
@@ -1311,27 +1473,43 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   float hgx = 0.f, hgy = 0.f, hgz = 0.f;
   bool hsame = false;
-  auto add_term = [&](bool same, float gx, float gy, float gz, float omj) {
-    const float co = omj - om.w;
-    if (same && isfinite(co)) return;
-    ex += co * gx;
-    ey += co * gy;
-    ez += co * gz;
+  // what a neighbor with |omega| = omj adds for a particle with |omega| = ws
+  auto eterm = [&](float ws, bool same, float gx, float gy, float gz, float omj) {
+    const float co = omj - ws;
+    if (same && isfinite(co)) return V3Term{0.0f, 0.0f, 0.0f, 0};
+    return V3Term{co * gx, co * gy, co * gz, 1};
   };
+  auto term = [&](const float3& xs, float ws, const EPosW& q) {
+    const float dx = xs.x - q.p.x;
+    const float dy = xs.y - q.p.y;
+    const float dz = xs.z - q.p.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < c.h2)) return V3Term{0.0f, 0.0f, 0.0f, 0};
+    const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
+    const float f = pair_spiky(r2, c);
+    return eterm(ws, same, f * dx, f * dy, f * dz, q.w);
+  };
+  auto add = [&](const V3Term& t) {
+    if (!t.on) return;
+    ex += t.x;
+    ey += t.y;
+    ez += t.z;
+  };
+  float3 gx3 = xi;  // the group's position and |omega|
+  float gw = om.w;
   gather(
       c, nv, cv, L,
       [&](int t) { return EPosW{ld3(pos, t), reinterpret_cast<const float*>(omega + t)[3]}; },
-      [&](const EPosW& q) {
-        const float dx = xi.x - q.p.x;
-        const float dy = xi.y - q.p.y;
-        const float dz = xi.z - q.p.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < c.h2) {
-          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-          const float f = pair_spiky(r2, c);
-          add_term(same, f * dx, f * dy, f * dz, q.w);
-        }
-      },
+      pair_ops(
+          [&](int leader) {
+            return same_bits(bcast(xi, leader), xi) && same_bits(bcast(om.w, leader), om.w);
+          },
+          [&](int leader) {
+            gx3 = bcast(xi, leader);
+            gw = bcast(om.w, leader);
+          },
+          [&](const EPosW& q) { return term(gx3, gw, q); }, add,
+          [&](const EPosW& q) { add(term(xi, om.w, q)); }),
       [&](const EPosW& tile, int m) {
         const int lane = threadIdx.x & (kWave - 1);
         return tile_all(lane >= m || fabsf(tile.w) < kHalfMax);
@@ -1348,7 +1526,7 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
         hgx = f * dx, hgy = f * dy, hgz = f * dz;
         return 0;
       },
-      [&](const EPosW& q) { add_term(hsame, hgx, hgy, hgz, q.w); }, [&](int) {});
+      [&](const EPosW& q) { add(eterm(om.w, hsame, hgx, hgy, hgz, q.w)); }, [&](int) {});
   if (!L.active) return;
   const float len = sqrtf(ex * ex + ey * ey + ez * ez);
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;

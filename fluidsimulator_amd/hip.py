@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfluid_hip.so")
 MAX_PLANES = 8
 
 OK, EINVAL, ENODEV, EHIP, ECAPACITY, ERANGE, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5, -6, -7
+TUNE_WAVE_MODE, TUNE_BLOCK, TUNE_BRICK, TUNE_XCD = 1, 2, 4, 8
 
 
 class FluidHipParams(C.Structure):
@@ -82,8 +83,8 @@ _SIGS = {
 
 # Autotuning and the differentiable rollout.
 _OPTIONAL_SIGS: dict = {
-    "fluid_hip_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(FluidHipConfig),
-                                     C.POINTER(C.c_double)]),
+    "fluid_hip_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
+                                     C.POINTER(FluidHipConfig), C.POINTER(C.c_double)]),
     "fluid_hip_forward_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fluid_hip_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
@@ -95,11 +96,13 @@ def exported_symbols():
     return list(_SIGS) + list(_OPTIONAL_SIGS)
 
 
-def load_library(path: str = LIB_PATH):
-    """Load the HIP library; raises if it is not built."""
+def load_library(path: str | None = None):
+    """Load the HIP library (`path`, else $FLUID_HIP_LIBRARY, else the
+    in-tree build); raises if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("FLUID_HIP_LIBRARY") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built: run `python -m fluidsimulator_amd.build`")
     lib = C.CDLL(path)
@@ -278,13 +281,15 @@ class HipSolver:
                 "record_bytes", "recomputed")
         return dict(zip(keys, [int(x) for x in out]))
 
-    def autotune(self, trial_steps=2):
+    def autotune(self, trial_steps=2, knobs=0):
         """Time the result-preserving launch variants on the current state
-        and keep the fastest (the state is restored).  Returns the chosen
-        knobs and the timings."""
+        and keep the fastest (the state is restored).  knobs: mask of
+        TUNE_* (0: workgroup size, brick edge and XCD order).  Returns the
+        chosen knobs and the timings."""
         cfg = FluidHipConfig()
         rep = (C.c_double * 4)()
-        _check(self.lib.fluid_hip_autotune(self.h, int(trial_steps), C.byref(cfg), rep))
+        _check(self.lib.fluid_hip_autotune(self.h, int(trial_steps), int(knobs), C.byref(cfg),
+                                           rep))
         return {"block_size": cfg.block_size, "xcd_remap": cfg.xcd_remap,
                 "wave_mode": cfg.wave_mode, "brick_edge": cfg.brick_edge,
                 "start_ms_per_step": rep[0], "best_ms_per_step": rep[1],

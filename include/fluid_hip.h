@@ -181,18 +181,27 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
 /* ---- autotuning ---------------------------------------------------------- */
 
 /* No counterpart in the reference.  Picks the launch configuration of this
- * context by timing: the gather mode of wavefronts sharing a cell (wave_mode
- * 1..3), the workgroup size (64..1024), the cell table's brick edge (1, 2, 4,
- * 8) and the XCD-aware tile order, tuned one knob at a time over two passes.
- * Every candidate runs `trial_steps` substeps from the current state between
- * two events; its final state must equal the starting configuration's bit
- * for bit, or it is rejected.  The fastest candidate (by at least 3%) is
- * kept; the state and time are restored.  `best` receives the chosen
- * configuration, report[0..3] the starting and chosen ms per substep, the
- * candidates timed and the candidates rejected (either may be NULL).  A
- * recorded rollout is discarded. */
-int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, fluid_hip_config* best,
-                       double* report);
+ * context by timing, one knob at a time over two passes, among the knobs in
+ * `knobs` (a mask of FLUID_HIP_TUNE_*; 0 selects FLUID_HIP_TUNE_DEFAULT):
+ * the workgroup size (64..1024), the cell table's brick edge (1, 2, 4, 8),
+ * the XCD-aware tile order and, on request, the gather mode of wavefronts
+ * sharing a cell (wave_mode 1..3).  The gather mode is left out by default:
+ * which mode is fastest depends on how far particles have piled up, which a
+ * short trial at one state does not show.  Every candidate runs
+ * `trial_steps` substeps from the current state between two events; its
+ * final state must equal the starting configuration's bit for bit, or it is
+ * rejected.  The fastest candidate (by at least 3%) is kept; the state and
+ * time are restored.  `best` receives the chosen configuration, report[0..3]
+ * the starting and chosen ms per substep, the candidates timed and the
+ * candidates rejected (either may be NULL).  A recorded rollout is
+ * discarded. */
+#define FLUID_HIP_TUNE_WAVE_MODE 1
+#define FLUID_HIP_TUNE_BLOCK 2
+#define FLUID_HIP_TUNE_BRICK 4
+#define FLUID_HIP_TUNE_XCD 8
+#define FLUID_HIP_TUNE_DEFAULT (FLUID_HIP_TUNE_BLOCK | FLUID_HIP_TUNE_BRICK | FLUID_HIP_TUNE_XCD)
+int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
+                       fluid_hip_config* best, double* report);
 
 /* ---- differentiable rollout ---------------------------------------------- */
 

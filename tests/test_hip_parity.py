@@ -173,7 +173,8 @@ def test_autotune_restores_state_and_keeps_results():
         s.upload(sc.state.pos, sc.state.vel, 0.0)
         s.step(5)
         before = s.download()
-        rep = s.autotune(trial_steps=2)
+        rep = s.autotune(trial_steps=2, knobs=hip.TUNE_WAVE_MODE | hip.TUNE_BLOCK |
+                         hip.TUNE_BRICK | hip.TUNE_XCD)
         after = s.download()
         assert same_bits(before.pos, after.pos) and same_bits(before.vel, after.vel)
         assert after.time == before.time
