@@ -333,6 +333,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--brick-edge", type=int, default=0, help="cell table brick edge (0: 4)")
     ap.add_argument("--dims", type=str, default="128,128,64")
     ap.add_argument("--workload", choices=["million_block", "ensemble", "adjoint"],
                     default="million_block",
@@ -374,7 +375,8 @@ def main():
     dims =tuple(int(x) for x in args.dims.split(","))
     sc = scenes.million_block(dims=dims)
     n = sc.state.n
-    s = hip.HipSolver(sc.params, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap))
+    s = hip.HipSolver(sc.params, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap),
+                      brick_edge=args.brick_edge)
     s.upload(sc.state.pos, sc.state.vel, 0.0)
     s.step(args.warmup, graph=args.graph)
     start_state = s.download() if not args.no_cpu else None
@@ -444,7 +446,7 @@ def main():
                    "block": tuned["block_size"] if tuned else args.block,
                    "xcd_remap": bool(tuned["xcd_remap"] if tuned else args.xcd_remap),
                    "wave_mode": tuned["wave_mode"] if tuned else 2,
-                   "brick_edge": tuned["brick_edge"] if tuned else 4},
+                   "brick_edge": tuned["brick_edge"] if tuned else (args.brick_edge or 4)},
         "autotune": tuned,
         "particle_steps_per_s": steps_per_s * n,
         "step_hbm_fraction": step_gbs / HBM_PEAK_GBS,
