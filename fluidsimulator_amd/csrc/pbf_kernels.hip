@@ -109,6 +109,19 @@ __device__ __forceinline__ float4 bcast(const float4& v, int j) {
   return make_float4(bcast(v.x, j), bcast(v.y, j), bcast(v.z, j), bcast(v.w, j));
 }
 
+// Bitwise equality of float values (distinguishes -0 and +0, NaN equals itself).
+__device__ __forceinline__ bool same_bits(const float3& a, const float3& b) {
+  return __float_as_int(a.x) == __float_as_int(b.x) && __float_as_int(a.y) == __float_as_int(b.y) &&
+         __float_as_int(a.z) == __float_as_int(b.z);
+}
+__device__ __forceinline__ bool same_bits(const float4& a, const float4& b) {
+  return same_bits(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z)) &&
+         __float_as_int(a.w) == __float_as_int(b.w);
+}
+__device__ __forceinline__ bool same_bits(float a, float b) {
+  return __float_as_int(a) == __float_as_int(b);
+}
+
 // ---------------------------------------------------------------- transfers
 
 __global__ void k_upload(int n, const float* __restrict__ pos, const float* __restrict__ vel,
@@ -709,10 +722,17 @@ __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
 __device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
 
 template <bool kFill>
-__device__ __forceinline__ void group_walk_cells(const Group& gr, bool mine, int s, float px,
-                                                float py, float pz, const StepConsts& c,
+__device__ __forceinline__ void group_walk_cells(const Group& gr, int leader, bool mine, int s,
+                                                float px, float py, float pz,
+                                                const StepConsts& c,
                                                 const int* __restrict__ start,
                                                 const float4* __restrict__ pred, RunWriter& w) {
+  // A group at one point (bit for bit) tests each candidate once: lane j
+  // tests candidate j and the outcomes form one mask for the whole group.
+  const float3 me = make_float3(px, py, pz);
+  const float3 gp = bcast(me, leader);  // whole wavefront
+  const bool uni = c.coop_bulk && __all(!mine || same_bits(gp, me));
+  const int lane = threadIdx.x & (kWave - 1);
   group_tiles(gr, start, [&](int t) { return ld3(pred, t); },
               [&](int base, int m, const float3& tile) {
                 float3 cp;
@@ -730,6 +750,25 @@ __device__ __forceinline__ void group_walk_cells(const Group& gr, bool mine, int
                       } else {
                         w.add_range<kFill>(base, base + m);
                       }
+                    }
+                  }
+                  return;
+                }
+                if (uni) {
+                  const float ex = gp.x - tile.x;
+                  const float ey = gp.y - tile.y;
+                  const float ez = gp.z - tile.z;
+                  const bool in = lane < m && ex * ex + ey * ey + ez * ez < c.h2;
+                  unsigned long long mask = __ballot(in);
+                  if (mine) {
+                    if (s >= base && s < base + m) mask &= ~(1ull << (s - base));
+                    // the mask's runs of set bits, in order
+                    while (mask != 0ull) {
+                      const int a = __builtin_ctzll(mask);
+                      const unsigned long long rest = ~(mask >> a);
+                      const int len = rest != 0ull ? __builtin_ctzll(rest) : kWave - a;
+                      w.add_range<kFill>(base + a, base + a + len);
+                      mask = (a + len >= kWave) ? 0ull : (mask & (~0ull << (a + len)));
                     }
                   }
                   return;
@@ -794,8 +833,8 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   RunWriter w{nullptr};
   if (c.coop)
     for_each_key_group(dense, L.key, [&](int leader, bool mine) {
-      group_walk_cells<false>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
-                              pred, w);
+      group_walk_cells<false>(group_of(c, cv, L, leader), leader, mine, s, p.x, p.y, p.z, c,
+                              cv.start, pred, w);
     });
   if (active && !dense) walk_cells<false>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, w);
   // The encoded length of one list stays below 2^31 words (at most two words
@@ -813,8 +852,8 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   RunWriter f{nbr + (ok ? off : 0)};
   if (c.coop)
     for_each_key_group(dense && ok, L.key, [&](int leader, bool mine) {
-      group_walk_cells<true>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
-                             pred, f);
+      group_walk_cells<true>(group_of(c, cv, L, leader), leader, mine, s, p.x, p.y, p.z, c,
+                             cv.start, pred, f);
     });
   if (ok && !dense) walk_cells<true>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, f);
   if (L.active) {
@@ -882,17 +921,6 @@ __device__ __forceinline__ PairOps<Same, Bind, GTerm, Add, Pair> pair_ops(Same s
   return {s, b, g, a, p};
 }
 
-__device__ __forceinline__ bool same_bits(const float3& a, const float3& b) {
-  return __float_as_int(a.x) == __float_as_int(b.x) && __float_as_int(a.y) == __float_as_int(b.y) &&
-         __float_as_int(a.z) == __float_as_int(b.z);
-}
-__device__ __forceinline__ bool same_bits(const float4& a, const float4& b) {
-  return same_bits(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z)) &&
-         __float_as_int(a.w) == __float_as_int(b.w);
-}
-__device__ __forceinline__ bool same_bits(float a, float b) {
-  return __float_as_int(a) == __float_as_int(b);
-}
 
 // Lanes of one group that share their pair state bit for bit form a class;
 // a group with at most this many classes walks its tiles once per class
