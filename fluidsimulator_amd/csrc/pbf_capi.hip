@@ -159,6 +159,9 @@ void compute_consts(Ctx& c) {
   k.xcd_remap = c.xcd_remap;
   k.coop = c.wave_mode >= 2 ? 1 : 0;
   k.coop_bulk = c.wave_mode == 2 ? 1 : 0;
+  k.dedup = (c.wave_mode == 2 && !c.record_diag) ? 1 : 0;
+  k.full_lists = c.record_diag ? 1 : 0;
+  k.count_only = 0;
   c.consts = k;
   c.iterations = p.solver_iterations;
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423
@@ -213,13 +216,14 @@ void free_adj(Ctx& c);
 void free_all(Ctx& c) {
   drop_graph(c);
   free_adj(c);
-  for (float4** p : {&c.posA, &c.velA, &c.posB, &c.velB, &c.pred[0], &c.pred[1], &c.omega})
+  for (float4** p : {&c.posA, &c.velA, &c.posB, &c.velB, &c.pred[0], &c.pred[1], &c.pred[2],
+                    &c.omega})
     dfree(*p);
   dfree(c.rho);
   dfree(c.cellc);
   for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
                   &c.total_cells, &c.scan_partial, &c.slot_key, &c.nbr, &c.seg, &c.nwords,
-                  &c.ncount, &c.tile_bbox,
+                  &c.ncount, &c.tile_bbox, &c.wave_link, &c.wave_head,
                   &c.steps_started})
     dfree(*p);
   dfree(c.slot_cell);
@@ -229,6 +233,7 @@ void free_all(Ctx& c) {
   dfree(c.ncount);
   dfree(c.nbr_total);
   dfree(c.need_cells);
+  dfree(c.pair_total);
   dfree(c.planes);
   dfree(c.flags);
   dfree(c.stage);
@@ -592,6 +597,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.velB, n);
   ALLOC(c.pred[0], n);
   ALLOC(c.pred[1], n);
+  ALLOC(c.pred[2], n);
   if (need_omega) ALLOC(c.omega, n);
   ALLOC(c.rho, n);
   ALLOC(c.cellc, n);
@@ -607,11 +613,14 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.ncount, n);
   ALLOC(c.bbox, 6 * static_cast<size_t>(n_scenes));
   ALLOC(c.tile_bbox, 6 * (n / 64 + 1));
+  ALLOC(c.wave_link, n / 64 + 2);
+  ALLOC(c.wave_head, n / 64 + 2);
   ALLOC(c.grid, static_cast<size_t>(n_scenes));
   ALLOC(c.total_cells, 1);
   ALLOC(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
   ALLOC(c.nbr_total, 1);
   ALLOC(c.need_cells, 1);
+  ALLOC(c.pair_total, 1);
   ALLOC(c.seg, static_cast<size_t>(n_scenes) + 1);
   ALLOC(c.planes, static_cast<size_t>(n_scenes));
   ALLOC(c.flags, 1);
@@ -831,7 +840,7 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
   if (lambda) {
     if (c.iterations > 0) {
       std::vector<float4> P(n);
-      HIP_TRY(hipMemcpy(P.data(), c.pred[(c.iterations - 1) & 1], sizeof(float4) * n,
+      HIP_TRY(hipMemcpy(P.data(), fluidhip::iterate_buffer(c, c.iterations - 1), sizeof(float4) * n,
                         hipMemcpyDeviceToHost),
               "read lambda");
       for (size_t s = 0; s < n; ++s) lambda[ids[s]] = P[s].w;
@@ -854,6 +863,11 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
   HIP_TRY(hipMemcpy(pos.data(), c.posA, sizeof(float4) * n, hipMemcpyDeviceToHost), "read pos");
   HIP_TRY(hipMemcpy(off.data(), c.noff, sizeof(long long) * n, hipMemcpyDeviceToHost), "read offsets");
   HIP_TRY(hipMemcpy(nw.data(), c.nwords, sizeof(int) * n, hipMemcpyDeviceToHost), "read lengths");
+  for (int x : nw)
+    if (x < 0)
+      return fail(FLUID_HIP_ESTATE,
+                  "the last step stored no list for some particles; create the context "
+                  "with record_diag for complete lists");
   unsigned long long used = 0;
   HIP_TRY(hipMemcpy(&used, c.nbr_total, sizeof(used), hipMemcpyDeviceToHost), "read total");
   used = std::min<unsigned long long>(used, static_cast<unsigned long long>(c.nbr_cap));
@@ -916,9 +930,21 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
   int64_t v[8] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events, tape_bytes(c),
                   c.adj ? c.adj->recomputed : 0};
   if (c.stepped && c.n > 0) {
-    std::vector<int> cnt(c.n);
-    HIP_TRY(hipMemcpy(cnt.data(), c.ncount, sizeof(int) * c.n, hipMemcpyDeviceToHost), "read counts");
-    for (int x : cnt) v[0] += x;
+    if (c.lists_partial) {
+      // cooperative particles stored no list: count their pairs again
+      HIP_TRY(fluidhip::enqueue_count_pairs(c), "count pairs");
+      unsigned long long pairs = 0;
+      HIP_TRY(hipMemcpyAsync(&pairs, c.pair_total, sizeof(pairs), hipMemcpyDeviceToHost,
+                             c.stream),
+              "read pairs");
+      HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+      v[0] = static_cast<int64_t>(pairs);
+    } else {
+      std::vector<int> cnt(c.n);
+      HIP_TRY(hipMemcpy(cnt.data(), c.ncount, sizeof(int) * c.n, hipMemcpyDeviceToHost),
+              "read counts");
+      for (int x : cnt) v[0] += x;
+    }
     unsigned long long words = 0;
     HIP_TRY(hipMemcpy(&words, c.nbr_total, sizeof(words), hipMemcpyDeviceToHost), "read words");
     v[1] = static_cast<int64_t>(words);

@@ -51,7 +51,10 @@ struct Ctx {
   float4* velA = nullptr;  // xyz, w = rho of the last lambda pass (scratch)
   float4* posB = nullptr;
   float4* velB = nullptr;
-  float4* pred[2] = {nullptr, nullptr};  // xyz, w = lambda
+  // constraint iterates, xyz, w = lambda: pred[0] holds p^0 (the predicted
+  // positions the neighbor lists are built from) for the whole substep,
+  // later iterates alternate between pred[1] and pred[2]
+  float4* pred[3] = {nullptr, nullptr, nullptr};
   float4* omega = nullptr;               // xyz = curl, w = |curl|
   float* rho = nullptr;
 
@@ -72,6 +75,8 @@ struct Ctx {
   int* scan_partial = nullptr;  // scan_blocks + 1
   uint64_t* slot_cell = nullptr;  // packed cell of pred0 per sorted slot
   int* slot_key = nullptr;  // table key of each sorted slot's cell
+  int* wave_link = nullptr;  // per 64-slot wave: repeats the wave before it
+  int* wave_head = nullptr;  // per wave: first wave of its chain of repeats
 
   // neighbors
   long long* noff = nullptr;  // first word of each slot's list
@@ -81,6 +86,7 @@ struct Ctx {
   long long nbr_cap = 0;
   unsigned long long* nbr_total = nullptr;
   unsigned long long* need_cells = nullptr;  // cells a failed substep needed
+  unsigned long long* pair_total = nullptr;  // neighbor pairs, counted on request
 
   // scenes
   int* seg = nullptr;
@@ -93,6 +99,9 @@ struct Ctx {
   bool uploaded = false;
   bool stepped = false;
   long long grow_events = 0;  // capacity growths (each replays the failed substep)
+  // the last substep stored no lists for cooperative particles (their
+  // neighbor counts need a counting walk), and its p^0 is in pred[0]
+  bool lists_partial = false;
 
   // staging for transfers
   float* stage = nullptr;  // 6 * n
@@ -115,6 +124,11 @@ struct SubstepRec {
   unsigned* const* M;  // planes that pushed p^k + delta
   unsigned* RM;        // restitution: planes in contact | approached << 8
 };
+
+// Array holding iterate k of the constraint loop (unrecorded substeps).
+inline float4* iterate_buffer(Ctx& c, int k) {
+  return k == 0 ? c.pred[0] : c.pred[1 + ((k - 1) & 1)];
+}
 
 // Queues one substep on ctx.stream (kernels in pbf_kernels.hip); with `rec`
 // the iterates and branch decisions are also written to the recording.
@@ -177,6 +191,9 @@ hipError_t enqueue_adjoint_result(Ctx& c, float* d_dpos, float* d_dvel, const Ad
 hipError_t enqueue_upload(Ctx& c, const float* d_pos, const float* d_vel);
 hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel);
 hipError_t enqueue_reset_flags(Ctx& c);
+// Counts the neighbor pairs of the last substep's grid (lists of
+// cooperative particles are not stored) into c.pair_total.
+hipError_t enqueue_count_pairs(Ctx& c);
 hipError_t enqueue_init_grid(Ctx& c);
 
 }  // namespace fluidhip

@@ -156,6 +156,10 @@ __global__ void k_reset_flags(unsigned* flags, int* steps_started) {
   }
 }
 
+__global__ void k_reset_total(unsigned long long* total) {
+  if (threadIdx.x == 0) *total = 0ull;
+}
+
 __global__ void k_init_bbox(int n_scenes, int* bbox) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_scenes) return;
@@ -354,6 +358,11 @@ constexpr int kDenseCell = 32;
 // ... and so are lanes with at least this many neighbors (or, while the
 // lists are built, candidates).
 constexpr int kHeavyList = 256;
+
+// Distinct predicted positions p^0, among the lanes of a walk or among the
+// candidates of a tile, up to which a tile's neighbor sets are decided with
+// one test per position; beyond it each lane tests each candidate.
+constexpr int kP0Classes = 4;
 
 // Calls fn(leader, mine) once per distinct key among the lanes with
 // want == true: `leader` is the lowest such lane holding the key and `mine`
@@ -595,6 +604,17 @@ struct RunWriter {
   }
 };
 
+// The neighbor predicate: candidate q is a neighbor of a particle at p when
+// |p - q|^2 < h^2, both at their predicted positions p^0 (core.cpp:140-142
+// and 222-245).  Lists are built with it, and cooperative walks retest it
+// instead of reading stored lists, so both must use this one expression.
+__device__ __forceinline__ bool near0(const float3& p, const float3& q, const StepConsts& c) {
+  const float ex = p.x - q.x;
+  const float ey = p.y - q.y;
+  const float ez = p.z - q.z;
+  return ex * ex + ey * ey + ez * ez < c.h2;
+}
+
 // Candidates of a particle: the 27 cells around its own in (dz, dy, dx)
 // order, each cell's slots in ascending order, i.e. ascending original id.
 template <bool kFill>
@@ -610,12 +630,7 @@ __device__ __forceinline__ void walk_cells(int s, float px, float py, float pz, 
         const int b = start[k + 1];
         for (int t = a; t < b; ++t) {
           if (t == s) continue;
-          const float3 q = ld3(pred, t);
-          const float ex = px - q.x;
-          const float ey = py - q.y;
-          const float ez = pz - q.z;
-          const float r2 = ex * ex + ey * ey + ez * ez;
-          if (r2 < c.h2) w.add<kFill>(t);
+          if (near0(make_float3(px, py, pz), ld3(pred, t), c)) w.add<kFill>(t);
         }
       }
   w.flush<kFill>();
@@ -628,10 +643,15 @@ struct CellView {
   const GridDesc* __restrict__ grid;
   const int* __restrict__ start;
   const int* __restrict__ seg;
+  const int* __restrict__ wave_head;       // first wave of each wave's chain of repeats
+  const float4* __restrict__ p0;           // predicted positions p^0 (xyz) of the lists
 };
 
 // One lane's view: its slot (the last particle for lanes past the end, so
-// that every wavefront stays whole), cell and whether that cell is dense.
+// that every wavefront stays whole), cell and whether it walks
+// cooperatively ("dense").  A lane is inactive past the end and in a wave
+// that repeats the wave before it (its outputs are copied from the chain's
+// first wave).
 struct LaneCell {
   int s;
   bool active;
@@ -641,15 +661,18 @@ struct LaneCell {
 };
 
 // `heavy` adds lanes of sparse cells whose own walk is long (next to a
-// dense cell): they are grouped by their cell too.
+// dense cell): they are grouped by their cell too.  A particle without a
+// stored list (nwords < 0) always walks cooperatively.
 __device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellView& cv, int s0,
-                                              const int* __restrict__ ncount = nullptr) {
+                                              const int* __restrict__ ncount = nullptr,
+                                              const int* __restrict__ nwords = nullptr) {
   LaneCell L;
-  L.active = s0 < c.n;
-  L.s = L.active ? s0 : c.n - 1;
+  L.s = s0 < c.n ? s0 : c.n - 1;
+  L.active = s0 < c.n && !(c.dedup && cv.wave_head[s0 >> 6] != (s0 >> 6));
   L.key = cv.slot_key[L.s];
   L.cell = cv.slot_cell[L.s];
-  const bool heavy = ncount != nullptr && ncount[L.s] >= kHeavyList;
+  const bool heavy = (ncount != nullptr && ncount[L.s] >= kHeavyList) ||
+                     (nwords != nullptr && nwords[L.s] < 0);
   L.dense = c.coop && L.active &&
             (heavy || (cv.start[L.key + 1] - cv.start[L.key]) >= kDenseCell);
   return L;
@@ -721,67 +744,119 @@ __device__ __forceinline__ bool tile_one_point(const T& q, int m, float3& cp) {
 __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
 __device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
 
+// Lanes of a walk grouped by their p^0 (bit for bit).  Lanes that share p^0
+// share their neighbors, but for themselves.
+struct LaneClasses {
+  float3 p[kP0Classes];  // wavefront-uniform
+  int cls;               // this lane's class, -1 when none
+  int n;                 // classes found
+  bool all;              // every walking lane has a class
+};
+
+__device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, bool on) {
+  LaneClasses lc;
+  lc.cls = -1;
+  lc.n = 0;
+  bool pend = in;
+  if (on) {
+#pragma unroll
+    for (int k = 0; k < kP0Classes; ++k) {
+      const unsigned long long pm = __ballot(pend);
+      lc.p[k] = bcast(me0, pm != 0ull ? __builtin_ctzll(pm) : 0);
+      const bool eq = pend && same_bits(me0, lc.p[k]);
+      if (eq) lc.cls = k;
+      pend = pend && !eq;
+      lc.n += pm != 0ull ? 1 : 0;
+    }
+  }
+  lc.all = __ballot(pend) == 0ull;
+  return lc;
+}
+
+// This lane's neighbors among the m candidates of a tile (bit j: candidate
+// base + j, whose p^0 this lane holds in t0), by the list predicate near0;
+// `any` receives the candidates that some walking lane takes (a superset).
+// With few lane classes one ballot per class decides the tile; otherwise,
+// with few distinct candidate positions, one test per position and lane;
+// otherwise every lane tests every candidate.
+__device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc,
+                                                           const float3& me0, bool in,
+                                                           const float3& t0, int m, int self,
+                                                           bool on, const StepConsts& c,
+                                                           unsigned long long& any) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long full = m == kWave ? ~0ull : ((1ull << m) - 1ull);
+  unsigned long long mine = 0ull;
+  any = 0ull;
+  bool done = false;
+  if (on && lc.all) {
+#pragma unroll
+    for (int k = 0; k < kP0Classes; ++k) {
+      if (k < lc.n) {
+        const unsigned long long mk = __ballot(lane < m && near0(lc.p[k], t0, c));
+        any |= mk;
+        if (lc.cls == k) mine = mk;
+      }
+    }
+    done = true;
+  }
+  if (on && !done) {
+    // distinct candidate positions of the tile
+    float3 cp[kP0Classes];
+    unsigned long long cm[kP0Classes];
+    int n = 0;
+    bool pend = lane < m;
+#pragma unroll
+    for (int k = 0; k < kP0Classes; ++k) {
+      const unsigned long long pm = __ballot(pend);
+      cp[k] = bcast(t0, pm != 0ull ? __builtin_ctzll(pm) : 0);
+      const bool eq = pend && same_bits(t0, cp[k]);
+      cm[k] = __ballot(eq);
+      pend = pend && !eq;
+      n += pm != 0ull ? 1 : 0;
+    }
+    if (__ballot(pend) == 0ull) {
+#pragma unroll
+      for (int k = 0; k < kP0Classes; ++k) {
+        if (k < n) {
+          const bool nk = in && near0(me0, cp[k], c);
+          if (nk) mine |= cm[k];
+          if (__ballot(nk) != 0ull) any |= cm[k];
+        }
+      }
+      done = true;
+    }
+  }
+  if (!done) {
+    any = full;
+    for (int j = 0; j < m; ++j)
+      if (near0(me0, bcast(t0, j), c)) mine |= 1ull << j;
+  }
+  if (!in) mine = 0ull;
+  if (self >= 0 && self < m) mine &= ~(1ull << self);
+  return mine;
+}
+
 template <bool kFill>
-__device__ __forceinline__ void group_walk_cells(const Group& gr, int leader, bool mine, int s,
+__device__ __forceinline__ void group_walk_cells(const Group& gr, bool mine, int s,
                                                 float px, float py, float pz,
                                                 const StepConsts& c,
                                                 const int* __restrict__ start,
                                                 const float4* __restrict__ pred, RunWriter& w) {
-  // A group at one point (bit for bit) tests each candidate once: lane j
-  // tests candidate j and the outcomes form one mask for the whole group.
   const float3 me = make_float3(px, py, pz);
-  const float3 gp = bcast(me, leader);  // whole wavefront
-  const bool uni = c.coop_bulk && __all(!mine || same_bits(gp, me));
-  const int lane = threadIdx.x & (kWave - 1);
+  const bool on = c.coop_bulk != 0;
+  const LaneClasses lc = lane_classes(me, mine, on);
   group_tiles(gr, start, [&](int t) { return ld3(pred, t); },
               [&](int base, int m, const float3& tile) {
-                float3 cp;
-                if (c.coop_bulk && tile_one_point(tile, m, cp)) {
-                  // one test decides the whole tile (all but the lane itself)
-                  if (mine) {
-                    const float ex = px - cp.x;
-                    const float ey = py - cp.y;
-                    const float ez = pz - cp.z;
-                    const float r2 = ex * ex + ey * ey + ez * ez;
-                    if (r2 < c.h2) {
-                      if (s >= base && s < base + m) {
-                        w.add_range<kFill>(base, s);
-                        w.add_range<kFill>(s + 1, base + m);
-                      } else {
-                        w.add_range<kFill>(base, base + m);
-                      }
-                    }
-                  }
-                  return;
-                }
-                if (uni) {
-                  const float ex = gp.x - tile.x;
-                  const float ey = gp.y - tile.y;
-                  const float ez = gp.z - tile.z;
-                  const bool in = lane < m && ex * ex + ey * ey + ez * ez < c.h2;
-                  unsigned long long mask = __ballot(in);
-                  if (mine) {
-                    if (s >= base && s < base + m) mask &= ~(1ull << (s - base));
-                    // the mask's runs of set bits, in order
-                    while (mask != 0ull) {
-                      const int a = __builtin_ctzll(mask);
-                      const unsigned long long rest = ~(mask >> a);
-                      const int len = rest != 0ull ? __builtin_ctzll(rest) : kWave - a;
-                      w.add_range<kFill>(base + a, base + a + len);
-                      mask = (a + len >= kWave) ? 0ull : (mask & (~0ull << (a + len)));
-                    }
-                  }
-                  return;
-                }
-                for (int j = 0; j < m; ++j) {
-                  const float3 q = bcast(tile, j);  // whole wavefront
-                  const int t = base + j;
-                  if (!mine || t == s) continue;
-                  const float ex = px - q.x;
-                  const float ey = py - q.y;
-                  const float ez = pz - q.z;
-                  const float r2 = ex * ex + ey * ey + ez * ez;
-                  if (r2 < c.h2) w.add<kFill>(t);
+                unsigned long long any;
+                unsigned long long mask = tile_members(lc, me, mine, tile, m, s - base, on, c, any);
+                // the mask's runs of set bits, in order
+                while (mask != 0ull) {
+                  const int a = __builtin_ctzll(mask);
+                  const unsigned long long rest = ~(mask >> a);
+                  const int len = rest != 0ull ? __builtin_ctzll(rest) : kWave - a;
+                  w.add_range<kFill>(base + a, base + a + len);
+                  mask = (a + len >= kWave) ? 0ull : (mask & (~0ull << (a + len)));
                 }
               });
   if (mine) w.flush<kFill>();
@@ -813,7 +888,7 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   const int ext = 1 << g.bshift;  // cells per brick edge
   if (active && (bad || rx < 1 || ry < 1 || rz < 1 || rx > ext * g.nbx - 2 ||
                  ry > ext * g.nby - 2 || rz > ext * g.nbz - 2)) {
-    atomicOr(flags, kFlagRange);
+    if (!c.count_only) atomicOr(flags, kFlagRange);
     active = false;
   }
   // lanes of sparse cells next to dense ones: long candidate walks
@@ -830,16 +905,29 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   }
   // every dense group lies inside the table: its cell is some valid lane's
   const bool dense = (L.dense || heavy) && active;
+  // Particles of dense cells store no list unless full lists are asked for:
+  // the passes always walk them cooperatively and retest near0 on p^0 tile
+  // by tile, in the same order.  Particles of sparse cells keep theirs: a
+  // heavy one (next to a dense cell) may still have a short list, which the
+  // passes then walk on their own.
+  const bool listed = active && (!L.dense || c.full_lists || c.count_only);
   RunWriter w{nullptr};
   if (c.coop)
-    for_each_key_group(dense, L.key, [&](int leader, bool mine) {
-      group_walk_cells<false>(group_of(c, cv, L, leader), leader, mine, s, p.x, p.y, p.z, c,
-                              cv.start, pred, w);
+    for_each_key_group(dense && listed, L.key, [&](int leader, bool mine) {
+      group_walk_cells<false>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
+                              pred, w);
     });
   if (active && !dense) walk_cells<false>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, w);
+  if (c.count_only) {
+    // statistics: add this wavefront's pairs to *nbr_total, store nothing
+    unsigned long long v = active ? static_cast<unsigned long long>(w.count) : 0ull;
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & (kWave - 1)) == 0 && v != 0ull) atomicAdd(nbr_total, v);
+    return;
+  }
   // The encoded length of one list stays below 2^31 words (at most two words
   // per neighbor and fewer than 2^29 particles).
-  const int nw = active ? static_cast<int>(w.words) : 0;
+  const int nw = listed ? static_cast<int>(w.words) : 0;
   int total;
   const int pre = block_excl_scan(nw, lds, &total);
   if (threadIdx.x == 0)
@@ -847,57 +935,29 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   __syncthreads();
   const long long off = base_sh + pre;
   const bool fits = off + nw <= nbr_cap;
-  if (active && !fits) atomicOr(flags, kFlagNeighbors);
-  const bool ok = active && fits;
+  if (listed && !fits) atomicOr(flags, kFlagNeighbors);
+  const bool ok = listed && fits;
   RunWriter f{nbr + (ok ? off : 0)};
   if (c.coop)
     for_each_key_group(dense && ok, L.key, [&](int leader, bool mine) {
-      group_walk_cells<true>(group_of(c, cv, L, leader), leader, mine, s, p.x, p.y, p.z, c,
-                             cv.start, pred, f);
+      group_walk_cells<true>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
+                             pred, f);
     });
   if (ok && !dense) walk_cells<true>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, f);
   if (L.active) {
+    // nwords = -1: no stored list, the passes walk this particle cooperatively
+    const bool unlisted = active && !listed;
     noff[s] = ok ? off : 0;
-    nwords[s] = ok ? nw : 0;
+    nwords[s] = ok ? nw : (unlisted ? -1 : 0);
     ncount[s] = ok ? f.count : 0;
+  } else if (s == tile_index(c.xcd_remap) * static_cast<int>(blockDim.x) +
+                     static_cast<int>(threadIdx.x)) {
+    // a repeated wave: it builds no list (it computes nothing)
+    noff[s] = 0;
+    nwords[s] = -1;
+    ncount[s] = 0;
   }
 }
-
-// A lane's position in its own list during a cooperative walk.
-struct ListCursor {
-  const int* __restrict__ L;
-  int nw, w, cur, end;
-  __device__ __forceinline__ void advance() {
-    if (w < nw) {
-      const int e = L[w++];
-      cur = e & 0x7fffffff;
-      end = cur + 1;
-      if (e < 0) end = cur + L[w++];
-    } else {
-      cur = INT_MAX;
-      end = INT_MAX;
-    }
-  }
-  // Consumes this lane's neighbors inside [base, base + m) and returns how
-  // many there were.  The cells are visited in (dz, dy, dx) order, not slot
-  // order: a run that starts below this tile belongs to a later cell.
-  __device__ __forceinline__ int take(int base, int m) {
-    const int hi = base + m;
-    int cnt = 0;
-    while (cur >= base && cur < hi) {
-      const int e2 = min(end, hi);
-      cnt += e2 - cur;
-      cur = e2;
-      if (cur == end) advance();
-    }
-    return cnt;
-  }
-  __device__ __forceinline__ bool hit(int t) {
-    if (t != cur) return false;
-    if (++cur == end) advance();
-    return true;
-  }
-};
 
 // The pair arithmetic of one gather pass as five operations:
 //   same(leader)  this lane's pair state (what its terms depend on besides
@@ -932,12 +992,22 @@ __device__ __forceinline__ PairOps<Same, Bind, GTerm, Add, Pair> pair_ops(Same s
 // walk 3.64; no class walks 3.99).
 constexpr int kMaxClasses = 1;
 
+// A candidate tile with the predicted positions p^0 of the same slots.
+template <typename T>
+struct TileP0 {
+  T q;
+  float3 p0;
+};
+
 // Gathers over the neighbors of every lane of a wavefront, in each lane's
 // list order:
 //  * lanes in dense cells: cooperative walks of each distinct dense cell of
 //    the wavefront; candidate tiles are loaded once (coalesced) and handed
-//    lane to lane with readlane; each lane takes exactly the candidates its
-//    own list holds;
+//    lane to lane with readlane.  A lane's neighbors in a tile are the
+//    candidates t != s with near0(p^0_s, p^0_t), the predicate its list is
+//    built with, so the walk visits exactly the list, in its order, without
+//    reading it; when the group shares one p^0 a single ballot decides
+//    every candidate for all of its lanes;
 //  * when every candidate of a tile sits at one point cp, prep(cp, flag)
 //    hoists the pair geometry once per lane and says how the lane's
 //    neighbors in the tile contribute: 0 one by one through pair_u,
@@ -956,44 +1026,56 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
                                        const LaneCell& L, Load&& load, Ops&& ops,
                                        TFlag&& tflag, Prep&& prep, PairU&& pair_u,
                                        OnCount&& on_count) {
+  const float3 me0 = ld3(cv.p0, L.s);
+  using Q = decltype(load(0));
   // one walk of the group `gr` by the lanes with `in` set
   auto walk = [&](const Group& gr, bool in, bool uni) {
-    ListCursor lc{nv.list + nv.off[L.s], nv.nw[L.s], 0, INT_MAX, INT_MAX};
-    if (in) lc.advance();
-    group_tiles(gr, cv.start, load, [&](int base, int m, const auto& tile) {
-      float3 cp;
-      if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
-        const bool tf = tflag(tile, m);
-        const int mode = in ? prep(cp, tf) : 1;
-        if (__all(mode != 0)) {
-          const int cnt = lc.take(base, m);
-          if (mode == 2) on_count(cnt);
-          return;
-        }
-        for (int j = 0; j < m; ++j) {
-          const auto q = bcast(tile, j);
-          if (lc.hit(base + j)) {
-            if (mode == 0)
-              pair_u(q);
-            else if (mode == 2)
-              on_count(1);
+    const bool on = c.coop_bulk != 0;
+    const LaneClasses lc = lane_classes(me0, in, on);
+    group_tiles(
+        gr, cv.start, [&](int t) { return TileP0<Q>{load(t), ld3(cv.p0, t)}; },
+        [&](int base, int m, const TileP0<Q>& tt) {
+          const Q& tile = tt.q;
+          // this lane's neighbors among the tile's candidates, and the
+          // candidates some lane takes
+          unsigned long long any;
+          const unsigned long long mine =
+              tile_members(lc, me0, in, tt.p0, m, L.s - base, on, c, any);
+          float3 cp;
+          if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
+            const bool tf = tflag(tile, m);
+            const int mode = mine != 0ull ? prep(cp, tf) : 1;
+            if (__all(mode != 0)) {
+              if (mode == 2) on_count(__popcll(mine));
+              return;
+            }
+            for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
+              const int j = __builtin_ctzll(b);
+              const auto q = bcast(tile, j);
+              if ((mine >> j) & 1ull) {
+                if (mode == 0)
+                  pair_u(q);
+                else if (mode == 2)
+                  on_count(1);
+              }
+            }
+            return;
           }
-        }
-        return;
-      }
-      if (uni) {
-        const auto t = ops.gterm(tile);
-        for (int j = 0; j < m; ++j) {
-          const auto tj = bcast(t, j);
-          if (lc.hit(base + j)) ops.add(tj);
-        }
-        return;
-      }
-      for (int j = 0; j < m; ++j) {
-        const auto q = bcast(tile, j);
-        if (lc.hit(base + j)) ops.pair(q);
-      }
-    });
+          if (uni) {
+            const auto t = ops.gterm(tile);
+            for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
+              const int j = __builtin_ctzll(b);
+              const auto tj = bcast(t, j);
+              if ((mine >> j) & 1ull) ops.add(tj);
+            }
+            return;
+          }
+          for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
+            const int j = __builtin_ctzll(b);
+            const auto q = bcast(tile, j);
+            if ((mine >> j) & 1ull) ops.pair(q);
+          }
+        });
   };
   if (c.coop)
     for_each_key_group(L.dense, L.key, [&](int leader, bool mine) {
@@ -1117,7 +1199,8 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
                                                  const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
-      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
+                nv.nw);
   const int s = L.s;
   const float3 p = ld3(P, s);
   float rho = 0.0f, gsx = 0.0f, gsy = 0.0f, gsz = 0.0f, sg2 = 0.0f;
@@ -1209,7 +1292,8 @@ __global__ __launch_bounds__(1024) void k_delta(
     const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
-      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
+                nv.nw);
   const int s = L.s;
   const float4 pi = P[s];
   float nx = pi.x, ny = pi.y, nz = pi.z;
@@ -1327,7 +1411,8 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
                                                const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
-      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
+                nv.nw);
   const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float4 vi = vin[s];
@@ -1416,7 +1501,8 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
                                                      const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
-      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
+                nv.nw);
   const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float3 vi = ld3(vel, s);
@@ -1494,7 +1580,8 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
                                                      const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const LaneCell L =
-      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count);
+      lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
+                nv.nw);
   const int s = L.s;
   const float3 xi = ld3(pos, s);
   const float4 om = omega[s];
@@ -1575,6 +1662,105 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
   vel[s] = make_float4(vx, vy, vz, 0.0f);
 }
 
+// ---------------------------------------------------------------- repeated waves
+//
+// Consecutive sorted slots whose particles sit at one position with one
+// velocity (bit for bit, after gravity), in one cell, have the same
+// candidates and the same tests: their lists differ only in which slot
+// each leaves out as itself.  When every slot between two of them is such
+// a particle too, the two lists hold the same values in the same order, so
+// every pass computes bit-identical results for them.  A chain of
+// consecutive 64-slot waves that are each uniform in that sense and share
+// one key is therefore computed by its first wave alone; the others skip
+// every pass (their lanes are inactive) and copy the first wave's outputs
+// after each pass (k_dedup_fill).
+
+// link[w] = 1 when wave w and wave w-1 are both uniform with the same key.
+__global__ __launch_bounds__(1024) void k_dedup_link(StepConsts c,
+                                                     const float4* __restrict__ posB,
+                                                     const float4* __restrict__ velB,
+                                                     const int* __restrict__ slot_key,
+                                                     int* __restrict__ link,
+                                                     const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = s0 >> 6;
+  const int lane = threadIdx.x & (kWave - 1);
+  // uniform: all 64 slots of the wave exist and equal its first slot
+  auto uniform = [&](int wave, float3& x0, float3& v0, int& k0) {
+    const int sl = wave * kWave + lane;
+    const bool in = sl < c.n;
+    const int t = in ? sl : c.n - 1;
+    const float3 x = ld3(posB, t);
+    const float3 v = ld3(velB, t);
+    const int k = slot_key[t];
+    x0 = bcast(x, 0);
+    v0 = bcast(v, 0);
+    k0 = bcast(k, 0);
+    return __all(in && same_bits(x, x0) && same_bits(v, v0) && k == k0);
+  };
+  float3 xa, va, xb, vb;
+  int ka, kb;
+  const bool ua = uniform(w, xa, va, ka);
+  bool l = false;
+  if (w > 0) {
+    const bool ub = uniform(w - 1, xb, vb, kb);
+    l = ua && ub && ka == kb && same_bits(xa, xb) && same_bits(va, vb);
+  }
+  const int nwaves = (c.n + kWave - 1) / kWave;
+  if (lane == 0 && w < nwaves) link[w] = l ? 1 : 0;
+}
+
+// head[w] = the first wave of w's chain: the last wave v <= w with
+// link[v] = 0 (one workgroup: a running max over each thread's span, then
+// a max-scan across threads).
+__global__ __launch_bounds__(1024) void k_dedup_heads(int nwaves, const int* __restrict__ link,
+                                                      int* __restrict__ head,
+                                                      const unsigned* __restrict__ flags) {
+  __shared__ int sh[1024];
+  if (fatal(flags)) return;
+  const int T = blockDim.x;
+  const int span = (nwaves + T - 1) / T;
+  const int a = threadIdx.x * span;
+  const int b = min(nwaves, a + span);
+  int run = -1;
+  for (int w = a; w < b; ++w)
+    if (!link[w]) run = w;
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    const int v = threadIdx.x >= static_cast<unsigned>(o) ? sh[threadIdx.x - o] : -1;
+    __syncthreads();
+    sh[threadIdx.x] = max(sh[threadIdx.x], v);
+    __syncthreads();
+  }
+  int cur = threadIdx.x > 0 ? sh[threadIdx.x - 1] : -1;
+  for (int w = a; w < b; ++w) {
+    if (!link[w]) cur = w;
+    head[w] = cur;
+  }
+}
+
+// Copies the chain's first wave's output into every repeated wave's slots.
+// With `ids`, w is each slot's own original id, ids[s].w (the committed
+// position carries the id of the slot's input position, which differs from
+// the first wave's).
+__global__ __launch_bounds__(1024) void k_dedup_fill(StepConsts c,
+                                                     const int* __restrict__ head,
+                                                     float4* __restrict__ a,
+                                                     const float4* __restrict__ ids,
+                                                     const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int w = s >> 6;
+  const int h = head[w];
+  if (h == w) return;
+  float4 v = a[h * kWave];
+  if (ids) v.w = ids[s].w;
+  a[s] = v;
+}
+
 // ---------------------------------------------------------------- launching
 
 struct Launcher {
@@ -1609,6 +1795,18 @@ inline int blocks_for(long long n, int bs) { return static_cast<int>((n + bs - 1
 
 }  // namespace
 
+hipError_t enqueue_count_pairs(Ctx& c) {
+  k_reset_total<<<1, 64, 0, c.stream>>>(c.pair_total);
+  if (c.n == 0) return hipGetLastError();
+  StepConsts k = c.consts;
+  k.count_only = 1;
+  k.dedup = 0;
+  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg, c.wave_head, c.pred[0]};
+  k_neighbors<<<blocks_for(c.n, c.block), c.block, 0, c.stream>>>(
+      k, c.pred[0], cv, c.noff, c.nwords, c.ncount, c.nbr, c.nbr_cap, c.pair_total, c.flags);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_reset_flags(Ctx& c) {
   k_reset_flags<<<1, 64, 0, c.stream>>>(c.flags, c.steps_started);
   return hipGetLastError();
@@ -1634,12 +1832,26 @@ hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel) {
 hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (c.n == 0) return hipSuccess;
   if (rec && c.iterations < 1) return hipErrorInvalidValue;
-  // iterate k of the constraint loop: the Jacobi ping-pong pair, or the
+  // iterate k of the constraint loop: p^0 stays in its own array for the
+  // whole substep and later iterates alternate between two others, or the
   // recording's own array per iterate
-  auto PK = [&](int k) { return rec ? rec->P[k] : c.pred[k & 1]; };
+  auto PK = [&](int k) { return rec ? rec->P[k] : iterate_buffer(c, k); };
   Launcher L{c};
-  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg};
-  const StepConsts k = c.consts;
+  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg, c.wave_head, PK(0)};
+  StepConsts k = c.consts;
+  // a recorded substep keeps every particle's own list (the adjoint reads them)
+  if (rec) {
+    k.dedup = 0;
+    k.full_lists = 1;
+  }
+  c.lists_partial = !rec && k.coop && !k.full_lists;
+  const bool dedup = k.dedup != 0;
+  const int nwaves = (c.n + kWave - 1) / kWave;
+  auto fill = [&](float4* a, const float4* ids) {
+    if (dedup)
+      k_dedup_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(k, c.wave_head, a, ids,
+                                                                c.flags);
+  };
   const int bs = c.block;
   const int nb = blocks_for(c.n, bs);
   hipStream_t st = c.stream;
@@ -1680,6 +1892,12 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                  c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
+  if (dedup)
+    FH_TRY(L("dedup_plan", [&] {
+      k_dedup_link<<<blocks_for(static_cast<long long>(nwaves) * kWave, 256), 256, 0, st>>>(
+          k, c.posB, c.velB, c.slot_key, c.wave_link, c.flags);
+      k_dedup_heads<<<1, 1024, 0, st>>>(nwaves, c.wave_link, c.wave_head, c.flags);
+    }));
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nb, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
@@ -1700,6 +1918,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     unsigned* rm = rec ? rec->RM : nullptr;
     FH_TRY(L("lambda", [&] {
       k_lambda<<<nb, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
+      fill(P, nullptr);
     }));
     FH_TRY(L("delta", [&] {
       if (!last)
@@ -1714,6 +1933,12 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
                                                       pm, nullptr, c.flags);
+      if (!last) {
+        fill(Pn, nullptr);
+      } else {
+        fill(c.posA, c.posB);  // each slot keeps its own id
+        fill(v1, nullptr);
+      }
     }));
   }
   if (I == 0) {
@@ -1726,6 +1951,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
                                                        nullptr, nullptr, c.flags);
+      fill(c.posA, c.posB);
+      fill(v1, nullptr);
     }));
   }
   if (c.xsph) {
@@ -1737,11 +1964,13 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
       else
         k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                          c.planes, nullptr, c.flags);
+      fill(c.velA, nullptr);
     }));
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
       k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
+      fill(c.omega, nullptr);
     }));
     FH_TRY(L("vorticity_apply", [&] {
       if (c.restitution_any)
@@ -1750,6 +1979,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
       else
         k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                                c.planes, c.flags);
+      fill(c.velA, nullptr);
     }));
   }
 #undef FH_TRY

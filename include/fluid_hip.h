@@ -162,7 +162,9 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
 
 /* Neighbor lists of the last step as original ids, particle by particle in
  * original order, each list in the reference's order (dz, dy, dx cells,
- * ascending id).  `cap` entries of `ids`; returns the total in *total. */
+ * ascending id).  `cap` entries of `ids`; returns the total in *total.
+ * Without record_diag, particles of dense cells store no list (their passes
+ * retest the neighbor predicate): then FLUID_HIP_ESTATE. */
 int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* ids, int64_t cap, int64_t* total);
 
 /* Sorted cell key (scene offset + brick-Z-order index) of each sorted slot and
@@ -170,7 +172,8 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* ids, int64_t cap, int64
  * int32 per scene: origin x,y,z, bricks x,y,z, cell offset, cell count. */
 int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
 
-/* Sizes of the last step: out[0] neighbor pairs (decoded), out[1] encoded
+/* Sizes of the last step: out[0] neighbor pairs (every particle's, counted
+ * by a walk of the grid when dense cells stored no lists), out[1] stored
  * list words, out[2] grid cells (all scenes), out[3] neighbor-list capacity
  * (words), out[4] cell-table capacity, out[5] capacity growths since the
  * context was created, out[6] device bytes held by the rollout recording

@@ -100,3 +100,53 @@ def test_all_particles_at_one_point():
             s.step(2)
             st = s.download()
         assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"]), WAVE_MODES[mode]
+
+
+@pytest.mark.parametrize("flags", [False, True])
+def test_repeated_waves_are_skipped_exactly(flags):
+    """The default gather mode skips 64-slot waves that repeat the wave before
+    them (same position, velocity and cell, bit for bit) and copies the first
+    wave's results.  Long runs of identical particles in one cell, broken by a
+    few others at nearby points (so that lists differ between runs), must
+    still give the reference's results bit for bit, with and without the
+    skip."""
+    p = scenes.base_params((1.0, 1.0, 1.0), flags)
+    rng = np.random.default_rng(11)
+    big = np.repeat(np.array([[0.512], [0.507], [0.503]], np.float32), 1200, axis=1)
+    others = np.array([[0.515, 0.509, 0.501], [0.52, 0.5, 0.51], [0.508, 0.514, 0.506]],
+                      np.float32).T
+    others = np.repeat(others, 3, axis=1)
+    far = (0.5 + 0.05 * rng.standard_normal((3, 400))).astype(np.float32)
+    pos = np.concatenate([big, others, far], axis=1)
+    order = rng.permutation(pos.shape[1])
+    pos = np.ascontiguousarray(pos[:, order])
+    vel = np.zeros_like(pos)
+    vel[1] = -0.5
+    o = oracle.step(p, pos, vel, nsteps=3)
+    for diag in (False, True):  # diagnostics turn the skip off
+        with hip.HipSolver(p, [pos.shape[1]], record_diag=diag) as s:
+            s.upload(pos, vel)
+            s.step(3)
+            st = s.download()
+        assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"]), diag
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("flags", [False, True])
+def test_coincident_clusters_bitwise_with_skips(flags, mode):
+    """Without diagnostics, particles of dense cells store no neighbor list
+    (their passes retest the distance predicate) and repeated waves are
+    skipped; results stay bitwise and the pair statistics still count every
+    particle's neighbors."""
+    p = scenes.base_params((1.0, 1.0, 1.0), flags)
+    pos = _clusters()
+    vel = np.zeros_like(pos)
+    vel[1] = -0.5
+    o = oracle.step(p, pos, vel, nsteps=3, diag=True)
+    with hip.HipSolver(p, [pos.shape[1]], wave_mode=mode) as s:
+        s.upload(pos, vel)
+        s.step(3)
+        st = s.download()
+        pairs = s.stats()["pairs"]
+    assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"])
+    assert pairs == int(o["nbr_count"].astype(np.int64).sum())
