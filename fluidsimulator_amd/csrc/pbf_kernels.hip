@@ -364,6 +364,10 @@ constexpr int kHeavyList = 256;
 // one test per position; beyond it each lane tests each candidate.
 constexpr int kP0Classes = 4;
 
+// Distinct positions of a tile's candidates up to which a pass walks the
+// tile as runs at one point (the pair geometry hoisted once per run).
+constexpr int kTilePoints = 4;
+
 // Calls fn(leader, mine) once per distinct key among the lanes with
 // want == true: `leader` is the lowest such lane holding the key and `mine`
 // marks the lanes that share it.  Wavefront-uniform control flow; every
@@ -727,20 +731,6 @@ __device__ __forceinline__ void group_tiles(const Group& gr, const int* __restri
       }
 }
 
-// True when the m candidates of a tile share one position (bitwise), which
-// is returned in cp; wavefront-uniform.
-template <typename T>
-__device__ __forceinline__ bool tile_one_point(const T& q, int m, float3& cp) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int x0 = __builtin_amdgcn_readlane(__float_as_int(q.x), 0);
-  const int y0 = __builtin_amdgcn_readlane(__float_as_int(q.y), 0);
-  const int z0 = __builtin_amdgcn_readlane(__float_as_int(q.z), 0);
-  cp = make_float3(__int_as_float(x0), __int_as_float(y0), __int_as_float(z0));
-  const bool eq = lane >= m || (__float_as_int(q.x) == x0 && __float_as_int(q.y) == y0 &&
-                                __float_as_int(q.z) == z0);
-  return __ballot(eq) == ~0ull;
-}
-
 __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
 __device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
 
@@ -771,6 +761,28 @@ __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, 
   }
   lc.all = __ballot(pend) == 0ull;
   return lc;
+}
+
+// The distinct positions (bit for bit) of the m candidates of a tile, up to
+// K of them: cp[k] and the mask of the candidates there, cm[k]; n is the
+// number found.  True when every candidate is at one of them.
+template <int K>
+__device__ __forceinline__ bool tile_points(const float3& q, int m, float3 (&cp)[K],
+                                            unsigned long long (&cm)[K], int& n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  bool pend = lane < m;
+  n = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const unsigned long long pm = __ballot(pend);
+    if (pm == 0ull) break;
+    cp[k] = bcast(q, __builtin_ctzll(pm));
+    const bool eq = pend && same_bits(q, cp[k]);
+    cm[k] = __ballot(eq);
+    pend = pend && !eq;
+    ++n;
+  }
+  return __ballot(pend) == 0ull;
 }
 
 // This lane's neighbors among the m candidates of a tile (bit j: candidate
@@ -805,17 +817,7 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
     float3 cp[kP0Classes];
     unsigned long long cm[kP0Classes];
     int n = 0;
-    bool pend = lane < m;
-#pragma unroll
-    for (int k = 0; k < kP0Classes; ++k) {
-      const unsigned long long pm = __ballot(pend);
-      cp[k] = bcast(t0, pm != 0ull ? __builtin_ctzll(pm) : 0);
-      const bool eq = pend && same_bits(t0, cp[k]);
-      cm[k] = __ballot(eq);
-      pend = pend && !eq;
-      n += pm != 0ull ? 1 : 0;
-    }
-    if (__ballot(pend) == 0ull) {
+    if (tile_points(t0, m, cp, cm, n)) {
 #pragma unroll
       for (int k = 0; k < kP0Classes; ++k) {
         if (k < n) {
@@ -1008,11 +1010,13 @@ struct TileP0 {
 //    built with, so the walk visits exactly the list, in its order, without
 //    reading it; when the group shares one p^0 a single ballot decides
 //    every candidate for all of its lanes;
-//  * when every candidate of a tile sits at one point cp, prep(cp, flag)
-//    hoists the pair geometry once per lane and says how the lane's
-//    neighbors in the tile contribute: 0 one by one through pair_u,
-//    1 nothing at all (exact signed zeros), 2 `count` identical terms
-//    (on_count); flag is tflag(tile, m), evaluated by the whole wavefront;
+//  * when the candidates of a tile sit at no more than kTilePoints points,
+//    the tile is walked as maximal runs of consecutive candidates at one
+//    point cp: for each run prep(cp, flag) hoists the pair geometry once
+//    per lane and says how the lane's neighbors in the run contribute:
+//    0 one by one through pair_u, 1 nothing at all (exact signed zeros),
+//    2 `count` identical terms (on_count); flag is tflag(tile, m), evaluated
+//    by the whole wavefront;
 //  * lanes that share their pair state bit for bit (particles piled up at
 //    one point) get the same term from each candidate: in their walk, lane
 //    j computes candidate j's term once and every lane adds the broadcast
@@ -1041,22 +1045,44 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
           unsigned long long any;
           const unsigned long long mine =
               tile_members(lc, me0, in, tt.p0, m, L.s - base, on, c, any);
-          float3 cp;
-          if (c.coop_bulk && tile_one_point(pos_of(tile.p), m, cp)) {
+          float3 cps[kTilePoints];
+          unsigned long long cms[kTilePoints];
+          int npts = 0;
+          if (c.coop_bulk && tile_points(pos_of(tile.p), m, cps, cms, npts)) {
+            // Candidates at a few points: walk maximal runs of consecutive
+            // candidates at one point, each like a one-point tile.
             const bool tf = tflag(tile, m);
-            const int mode = mine != 0ull ? prep(cp, tf) : 1;
-            if (__all(mode != 0)) {
-              if (mode == 2) on_count(__popcll(mine));
-              return;
-            }
-            for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
-              const int j = __builtin_ctzll(b);
-              const auto q = bcast(tile, j);
-              if ((mine >> j) & 1ull) {
-                if (mode == 0)
-                  pair_u(q);
-                else if (mode == 2)
-                  on_count(1);
+            unsigned long long rest = any;
+            while (rest != 0ull) {
+              const int j = __builtin_ctzll(rest);
+              float3 cpk = cps[0];
+              unsigned long long cmk = cms[0];
+#pragma unroll
+              for (int k = 1; k < kTilePoints; ++k)
+                if (k < npts && ((cms[k] >> j) & 1ull)) {
+                  cpk = cps[k];
+                  cmk = cms[k];
+                }
+              const unsigned long long other = rest & ~cmk;
+              const unsigned long long upto =
+                  other != 0ull ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull;
+              const unsigned long long run = rest & upto;
+              rest &= ~upto;
+              const unsigned long long mr = mine & run;
+              const int mode = mr != 0ull ? prep(cpk, tf) : 1;
+              if (__all(mode != 0)) {
+                if (mode == 2) on_count(__popcll(mr));
+                continue;
+              }
+              for (unsigned long long b = run; b != 0ull; b &= b - 1ull) {
+                const int jj = __builtin_ctzll(b);
+                const auto q = bcast(tile, jj);
+                if ((mr >> jj) & 1ull) {
+                  if (mode == 0)
+                    pair_u(q);
+                  else if (mode == 2)
+                    on_count(1);
+                }
               }
             }
             return;
