@@ -723,9 +723,14 @@ __device__ __forceinline__ void group_tiles(const Group& gr, const int* __restri
             cell_key(gr.rx + dx, gr.ry + dy, gr.rz + dz, gr.g));
         const int a = __builtin_amdgcn_readfirstlane(start[k]);
         const int b = __builtin_amdgcn_readfirstlane(start[k + 1]);
+        if (a >= b) continue;
+        // the next tile's load is issued before this tile is processed
+        auto next = load(a + min(lane, min(kWave, b - a) - 1));
         for (int base = a; base < b; base += kWave) {
           const int m = min(kWave, b - base);
-          const auto tile = load(base + min(lane, m - 1));
+          const auto tile = next;
+          const int nb = base + kWave;
+          if (nb < b) next = load(nb + min(lane, min(kWave, b - nb) - 1));
           body(base, m, tile);
         }
       }
