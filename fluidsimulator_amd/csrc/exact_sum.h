@@ -23,7 +23,9 @@
 
 namespace fluidhip {
 
-__host__ __device__ inline float add_repeat(float x, float w, int k) {
+// Not inlined: callers flush runs rarely, and inlining its double-precision
+// temporaries into long gather loops costs registers there.
+__host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, float w, int k) {
   if (k <= 0) return x;
   // a zero, infinite or NaN term: one addition fixes the result for good
   if (w == 0.0f || !std::isfinite(w) || !std::isfinite(x)) return x + w;

@@ -1022,6 +1022,10 @@ struct TileP0 {
 //    0 one by one through pair_u, 1 nothing at all (exact signed zeros),
 //    2 `count` identical terms (on_count); flag is tflag(tile, m), evaluated
 //    by the whole wavefront;
+//  * consecutive tiles whose candidates all hold one record (bit for bit,
+//    p^0 included) form one range: each lane only counts its members, and
+//    the range then contributes like one run at one point, with mode 0
+//    adding pair_u's term `count` times exactly (pair_rep);
 //  * lanes that share their pair state bit for bit (particles piled up at
 //    one point) get the same term from each candidate: in their walk, lane
 //    j computes candidate j's term once and every lane adds the broadcast
@@ -1030,21 +1034,56 @@ struct TileP0 {
 // Every lane belongs to exactly one walk, so each sum still runs over the
 // lane's list in order.
 template <typename Load, typename Ops, typename TFlag, typename Prep, typename PairU,
-          typename OnCount>
+          typename PairRep, typename OnCount>
 __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, const CellView& cv,
                                        const LaneCell& L, Load&& load, Ops&& ops,
                                        TFlag&& tflag, Prep&& prep, PairU&& pair_u,
-                                       OnCount&& on_count) {
+                                       PairRep&& pair_rep, OnCount&& on_count) {
   const float3 me0 = ld3(cv.p0, L.s);
   using Q = decltype(load(0));
   // one walk of the group `gr` by the lanes with `in` set
   auto walk = [&](const Group& gr, bool in, bool uni) {
     const bool on = c.coop_bulk != 0;
     const LaneClasses lc = lane_classes(me0, in, on);
+    // a pending range of candidates with one record: the record and its
+    // p^0 and tile flag (uniform), this lane's members in it
+    bool pend = false, pend_tf = false, pend_near = false;
+    Q pend_q{};
+    float3 pend_p0 = make_float3(0.0f, 0.0f, 0.0f);
+    int pend_cnt = 0;
+    auto flush = [&]() {
+      if (!pend) return;
+      pend = false;
+      const int mode = pend_cnt > 0 ? prep(pos_of(pend_q.p), pend_tf) : 1;
+      if (mode == 2)
+        on_count(pend_cnt);
+      else if (mode == 0)
+        pair_rep(pend_q, pend_cnt);
+    };
     group_tiles(
         gr, cv.start, [&](int t) { return TileP0<Q>{load(t), ld3(cv.p0, t)}; },
         [&](int base, int m, const TileP0<Q>& tt) {
           const Q& tile = tt.q;
+          // one record across the tile (then r, r0), continuing the pending range?
+          const int lane = threadIdx.x & (kWave - 1);
+          const Q r = bcast(tile, 0);
+          const float3 r0 = bcast(tt.p0, 0);
+          const bool one = on && __all(lane >= m || (same_rec(tile, r) && same_bits(tt.p0, r0)));
+          const bool cont = one && pend && same_rec(r, pend_q) && same_bits(r0, pend_p0);
+          if (!cont) flush();
+          if (one) {
+            if (!cont) {
+              pend = true;
+              pend_q = r;
+              pend_p0 = r0;
+              pend_tf = tflag(tile, m);
+              pend_near = in && near0(me0, r0, c);
+              pend_cnt = 0;
+            }
+            const int self = L.s - base;
+            if (pend_near) pend_cnt += m - ((self >= 0 && self < m) ? 1 : 0);
+            return;
+          }
           // this lane's neighbors among the tile's candidates, and the
           // candidates some lane takes
           unsigned long long any;
@@ -1107,6 +1146,7 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
             if ((mine >> j) & 1ull) ops.pair(q);
           }
         });
+    flush();
   };
   if (c.coop)
     for_each_key_group(L.dense, L.key, [&](int leader, bool mine) {
@@ -1186,6 +1226,21 @@ __device__ __forceinline__ EPosW bcast(const EPosW& e, int j) {
   return EPosW{bcast(e.p, j), bcast(e.w, j)};
 }
 
+// Bitwise equality of candidate records.
+__device__ __forceinline__ bool same_rec(const EPos& a, const EPos& b) { return same_bits(a.p, b.p); }
+__device__ __forceinline__ bool same_rec(const EPosL& a, const EPosL& b) {
+  return same_bits(a.p, b.p) && same_bits(a.l, b.l);
+}
+__device__ __forceinline__ bool same_rec(const EPosV& a, const EPosV& b) {
+  return same_bits(a.p, b.p) && same_bits(a.v, b.v) && same_bits(a.irho, b.irho);
+}
+__device__ __forceinline__ bool same_rec(const EPosV3& a, const EPosV3& b) {
+  return same_bits(a.p, b.p) && same_bits(a.v, b.v);
+}
+__device__ __forceinline__ bool same_rec(const EPosW& a, const EPosW& b) {
+  return same_bits(a.p, b.p) && same_bits(a.w, b.w);
+}
+
 __device__ __forceinline__ bool tile_all(bool ok) { return __ballot(ok) == ~0ull; }
 
 // Per-candidate terms of the passes (what one neighbor adds to the sums).
@@ -1204,6 +1259,14 @@ __device__ __forceinline__ LTerm bcast(const LTerm& t, int j) {
 __device__ __forceinline__ V3Term bcast(const V3Term& t, int j) {
   return V3Term{bcast(t.x, j), bcast(t.y, j), bcast(t.z, j), bcast(t.on, j)};
 }
+// k successive additions of one 3-vector term, exactly (exact_sum.h).
+__device__ __forceinline__ void add_rep3(float& x, float& y, float& z, const V3Term& t, int k) {
+  if (!t.on) return;
+  x = add_repeat(x, t.x, k);
+  y = add_repeat(y, t.y, k);
+  z = add_repeat(z, t.z, k);
+}
+
 __device__ __forceinline__ bool same_term(const LTerm& a, const LTerm& b) {
   return a.grad == b.grad && same_bits(a.w, b.w) && same_bits(a.gx, b.gx) &&
          same_bits(a.gy, b.gy) && same_bits(a.gz, b.gz) && same_bits(a.j2, b.j2);
@@ -1289,7 +1352,7 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
         tile_t = term(p, cp);
         return 2;
       },
-      [&](const EPos&) {},
+      [&](const EPos&) {}, [&](const EPos&, int) {},
       [&](int cnt) {
         // cnt identical neighbors: each accumulator receives the same term
         // cnt times, in order
@@ -1396,6 +1459,9 @@ __global__ __launch_bounds__(1024) void k_delta(
           return 0;
         },
         [&](const EPosL& q) { add(dterm(pi, hdx, hdy, hdz, hsame, hf, hsc, q.l)); },
+        [&](const EPosL& q, int k) {
+          add_rep3(ddx, ddy, ddz, dterm(pi, hdx, hdy, hdz, hsame, hf, hsc, q.l), k);
+        },
         [&](int) {});
     if (!L.active) return;
     ddx *= c.inv_density;
@@ -1511,7 +1577,8 @@ __global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __res
           return 1;
         return 0;
       },
-      [&](const EPosV& q) { add(xterm(vi3, hW, q)); }, [&](int) {});
+      [&](const EPosV& q) { add(xterm(vi3, hW, q)); },
+      [&](const EPosV& q, int k) { add_rep3(ax, ay, az, xterm(vi3, hW, q), k); }, [&](int) {});
   if (!L.active) return;
   float vx = vi.x + c.visc_c * ax;
   float vy = vi.y + c.visc_c * ay;
@@ -1595,7 +1662,9 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
         hgx = f * dx, hgy = f * dy, hgz = f * dz;
         return 0;
       },
-      [&](const EPosV3& q) { add(wterm(vi, hsame, hgx, hgy, hgz, q.v)); }, [&](int) {});
+      [&](const EPosV3& q) { add(wterm(vi, hsame, hgx, hgy, hgz, q.v)); },
+      [&](const EPosV3& q, int k) { add_rep3(ox, oy, oz, wterm(vi, hsame, hgx, hgy, hgz, q.v), k); },
+      [&](int) {});
   if (!L.active) return;
   omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
 }
@@ -1672,7 +1741,9 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
         hgx = f * dx, hgy = f * dy, hgz = f * dz;
         return 0;
       },
-      [&](const EPosW& q) { add(eterm(om.w, hsame, hgx, hgy, hgz, q.w)); }, [&](int) {});
+      [&](const EPosW& q) { add(eterm(om.w, hsame, hgx, hgy, hgz, q.w)); },
+      [&](const EPosW& q, int k) { add_rep3(ex, ey, ez, eterm(om.w, hsame, hgx, hgy, hgz, q.w), k); },
+      [&](int) {});
   if (!L.active) return;
   const float len = sqrtf(ex * ex + ey * ey + ez * ez);
   float nx = 0.0f, ny = 0.0f, nz = 0.0f;
