@@ -99,4 +99,14 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
   return neg ? -x : x;
 }
 
+// add_repeat for hot loops: short runs are added one by one in place, long
+// ones in O(binades) by the call.
+__host__ __device__ inline float add_k(float x, float w, int k) {
+  if (k <= 8) {
+    for (int i = 0; i < k; ++i) x += w;
+    return x;
+  }
+  return add_repeat(x, w, k);
+}
+
 }  // namespace fluidhip

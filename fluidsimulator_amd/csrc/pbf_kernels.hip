@@ -1262,9 +1262,9 @@ __device__ __forceinline__ V3Term bcast(const V3Term& t, int j) {
 // k successive additions of one 3-vector term, exactly (exact_sum.h).
 __device__ __forceinline__ void add_rep3(float& x, float& y, float& z, const V3Term& t, int k) {
   if (!t.on) return;
-  x = add_repeat(x, t.x, k);
-  y = add_repeat(y, t.y, k);
-  z = add_repeat(z, t.z, k);
+  x = add_k(x, t.x, k);
+  y = add_k(y, t.y, k);
+  z = add_k(z, t.z, k);
 }
 
 __device__ __forceinline__ bool same_term(const LTerm& a, const LTerm& b) {
@@ -1320,12 +1320,12 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
   int run_n = 0;
   auto flush = [&]() {
     if (run_n == 0) return;
-    rho = add_repeat(rho, run.w, run_n);
+    rho = add_k(rho, run.w, run_n);
     if (run.grad) {
-      gsx = add_repeat(gsx, run.gx, run_n);
-      gsy = add_repeat(gsy, run.gy, run_n);
-      gsz = add_repeat(gsz, run.gz, run_n);
-      sg2 = add_repeat(sg2, run.j2, run_n);
+      gsx = add_k(gsx, run.gx, run_n);
+      gsy = add_k(gsy, run.gy, run_n);
+      gsz = add_k(gsz, run.gz, run_n);
+      sg2 = add_k(sg2, run.j2, run_n);
     }
     run_n = 0;
   };

@@ -82,6 +82,13 @@ int main(int argc, char** argv) {
         std::printf("MISMATCH x=%a w=%a k=%d fast=%a plain=%a\n", x, w, k, a, b);
       ++fails;
     }
+    // the in-place wrapper, on both sides of its cut-over
+    const int ks = 1 + (k % 16);
+    if (bits(fluidhip::add_k(x, w, ks)) != bits(naive(x, w, ks)) ||
+        bits(fluidhip::add_k(x, w, k)) != bits(b)) {
+      if (fails < 10) std::printf("MISMATCH add_k x=%a w=%a k=%d\n", x, w, k);
+      ++fails;
+    }
   }
   // special values
   const float sp[] = {0.0f, -0.0f, INFINITY, -INFINITY, NAN, 1.0f, -1.0f, 0x1p-149f, 0x1.fffffep127f};
