@@ -159,7 +159,7 @@ void compute_consts(Ctx& c) {
   k.xcd_remap = c.xcd_remap;
   k.coop = c.wave_mode >= 2 ? 1 : 0;
   k.coop_bulk = c.wave_mode == 2 ? 1 : 0;
-  k.dedup = (c.wave_mode == 2 && !c.record_diag) ? 1 : 0;
+  k.runs = (c.wave_mode == 2 && !c.record_diag) ? 1 : 0;
   k.full_lists = c.record_diag ? 1 : 0;
   k.count_only = 0;
   c.consts = k;
@@ -223,7 +223,8 @@ void free_all(Ctx& c) {
   dfree(c.cellc);
   for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
                   &c.total_cells, &c.scan_partial, &c.slot_key, &c.nbr, &c.seg, &c.nwords,
-                  &c.ncount, &c.tile_bbox, &c.wave_link, &c.wave_head,
+                  &c.ncount, &c.tile_bbox, &c.run_no, &c.run_excl, &c.rep_list, &c.run_end,
+                  &c.n_dev,
                   &c.steps_started})
     dfree(*p);
   dfree(c.slot_cell);
@@ -613,8 +614,11 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.ncount, n);
   ALLOC(c.bbox, 6 * static_cast<size_t>(n_scenes));
   ALLOC(c.tile_bbox, 6 * (n / 64 + 1));
-  ALLOC(c.wave_link, n / 64 + 2);
-  ALLOC(c.wave_head, n / 64 + 2);
+  ALLOC(c.run_no, n);
+  ALLOC(c.run_excl, static_cast<size_t>(n) + 1);
+  ALLOC(c.rep_list, n);
+  ALLOC(c.run_end, n);
+  ALLOC(c.n_dev, 1);
   ALLOC(c.grid, static_cast<size_t>(n_scenes));
   ALLOC(c.total_cells, 1);
   ALLOC(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
@@ -627,6 +631,8 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.steps_started, 1);
   ALLOC(c.stage, 6 * n);
 #undef ALLOC
+  e = hipMemcpy(c.n_dev, &c.n, sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(particle count)"));
   // Defaults leave room for the spray and pile-up of violent scenes (cell
   // tables of tens of millions of cells, run-length lists well above the
   // particle count); both grow on demand.

@@ -75,8 +75,12 @@ struct Ctx {
   int* scan_partial = nullptr;  // scan_blocks + 1
   uint64_t* slot_cell = nullptr;  // packed cell of pred0 per sorted slot
   int* slot_key = nullptr;  // table key of each sorted slot's cell
-  int* wave_link = nullptr;  // per 64-slot wave: repeats the wave before it
-  int* wave_head = nullptr;  // per wave: first wave of its chain of repeats
+  // runs of identical consecutive slots (one lane computes each run)
+  int* run_no = nullptr;     // per slot: head flag, then the slot's run number
+  int* run_excl = nullptr;   // n + 1: exclusive scan of the head flags; [n] = runs
+  int* rep_list = nullptr;   // per run: its first slot
+  int* run_end = nullptr;    // per slot: one past the last slot of its run
+  int* n_dev = nullptr;      // the particle count, on the device (scan length)
 
   // neighbors
   long long* noff = nullptr;  // first word of each slot's list

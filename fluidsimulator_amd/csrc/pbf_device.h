@@ -50,7 +50,7 @@ struct StepConsts {
   int xcd_remap;
   int coop;         // 1: wavefronts sharing a cell use shared candidate tiles
   int coop_bulk;    // 1: ... and whole coincident tiles in one step
-  int dedup;        // 1: waves that repeat the wave before them are skipped
+  int runs;         // 1: one lane per run of identical consecutive slots
   int full_lists;   // 1: cooperative particles get stored lists too (otherwise
                     //    their passes retest the list's distance predicate)
   int count_only;   // 1: the neighbor walk only counts pairs (statistics)

@@ -647,15 +647,17 @@ struct CellView {
   const GridDesc* __restrict__ grid;
   const int* __restrict__ start;
   const int* __restrict__ seg;
-  const int* __restrict__ wave_head;       // first wave of each wave's chain of repeats
+  const int* __restrict__ rep;             // with runs: the first slot of each run
+  const int* __restrict__ n_reps;          // with runs: the number of runs
+  const int* __restrict__ run_end;         // with runs: per slot, one past its run's end
   const float4* __restrict__ p0;           // predicted positions p^0 (xyz) of the lists
 };
 
 // One lane's view: its slot (the last particle for lanes past the end, so
 // that every wavefront stays whole), cell and whether it walks
-// cooperatively ("dense").  A lane is inactive past the end and in a wave
-// that repeats the wave before it (its outputs are copied from the chain's
-// first wave).
+// cooperatively ("dense").  With runs, lane i computes the first slot of
+// run i (the run's other slots copy its outputs) and lanes past the last
+// run are inactive.
 struct LaneCell {
   int s;
   bool active;
@@ -671,8 +673,13 @@ __device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellVie
                                               const int* __restrict__ ncount = nullptr,
                                               const int* __restrict__ nwords = nullptr) {
   LaneCell L;
-  L.s = s0 < c.n ? s0 : c.n - 1;
-  L.active = s0 < c.n && !(c.dedup && cv.wave_head[s0 >> 6] != (s0 >> 6));
+  if (c.runs) {
+    L.active = s0 < *cv.n_reps;
+    L.s = L.active ? cv.rep[s0] : c.n - 1;
+  } else {
+    L.active = s0 < c.n;
+    L.s = L.active ? s0 : c.n - 1;
+  }
   L.key = cv.slot_key[L.s];
   L.cell = cv.slot_cell[L.s];
   const bool heavy = (ncount != nullptr && ncount[L.s] >= kHeavyList) ||
@@ -732,6 +739,48 @@ __device__ __forceinline__ void group_tiles(const Group& gr, const int* __restri
           const int nb = base + kWave;
           if (nb < b) next = load(nb + min(lane, min(kWave, b - nb) - 1));
           body(base, m, tile);
+        }
+      }
+}
+
+// Runs of identical slots at least this long are walked as one step.
+constexpr int kRunStep = 32;
+
+// Like group_tiles, but with run_end (runs on) a run of at least kRunStep
+// slots is one step: body(base, m, tile, true) with m the run's length and
+// lane 0 of `tile` holding slot base (every slot of the run holds the same
+// record).  Otherwise body(base, m, tile, false) for tiles of up to 64
+// candidates.  The next step's load is issued before a step is processed.
+template <typename Load, typename Body>
+__device__ __forceinline__ void group_steps(const Group& gr, const int* __restrict__ start,
+                                            const int* __restrict__ run_end, Load&& load,
+                                            Body&& body) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int k = __builtin_amdgcn_readfirstlane(
+            cell_key(gr.rx + dx, gr.ry + dy, gr.rz + dz, gr.g));
+        const int a = __builtin_amdgcn_readfirstlane(start[k]);
+        const int b = __builtin_amdgcn_readfirstlane(start[k + 1]);
+        if (a >= b) continue;
+        int t = a;
+        int et = run_end ? __builtin_amdgcn_readfirstlane(run_end[t]) : t;
+        auto next = load(t + min(lane, min(kWave, b - t) - 1));
+        while (t < b) {
+          const int er = min(et, b);
+          const bool whole = run_end != nullptr && er - t >= kRunStep;
+          const int m = whole ? er - t : min(kWave, b - t);
+          const auto cur = next;
+          const int tn = t + m;
+          int en = tn;
+          if (tn < b) {
+            if (run_end) en = __builtin_amdgcn_readfirstlane(run_end[tn]);
+            next = load(tn + min(lane, min(kWave, b - tn) - 1));
+          }
+          body(t, m, cur, whole);
+          t = tn;
+          et = en;
         }
       }
 }
@@ -957,12 +1006,6 @@ __global__ __launch_bounds__(1024) void k_neighbors(
     noff[s] = ok ? off : 0;
     nwords[s] = ok ? nw : (unlisted ? -1 : 0);
     ncount[s] = ok ? f.count : 0;
-  } else if (s == tile_index(c.xcd_remap) * static_cast<int>(blockDim.x) +
-                     static_cast<int>(threadIdx.x)) {
-    // a repeated wave: it builds no list (it computes nothing)
-    noff[s] = 0;
-    nwords[s] = -1;
-    ncount[s] = 0;
   }
 }
 
@@ -1060,15 +1103,16 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
       else if (mode == 0)
         pair_rep(pend_q, pend_cnt);
     };
-    group_tiles(
-        gr, cv.start, [&](int t) { return TileP0<Q>{load(t), ld3(cv.p0, t)}; },
-        [&](int base, int m, const TileP0<Q>& tt) {
+    group_steps(
+        gr, cv.start, cv.run_end, [&](int t) { return TileP0<Q>{load(t), ld3(cv.p0, t)}; },
+        [&](int base, int m, const TileP0<Q>& tt, bool whole) {
           const Q& tile = tt.q;
-          // one record across the tile (then r, r0), continuing the pending range?
+          // one record across the step (then r, r0), continuing the pending range?
           const int lane = threadIdx.x & (kWave - 1);
           const Q r = bcast(tile, 0);
           const float3 r0 = bcast(tt.p0, 0);
-          const bool one = on && __all(lane >= m || (same_rec(tile, r) && same_bits(tt.p0, r0)));
+          const bool one =
+              whole || (on && __all(lane >= m || (same_rec(tile, r) && same_bits(tt.p0, r0))));
           const bool cont = one && pend && same_rec(r, pend_q) && same_bits(r0, pend_p0);
           if (!cont) flush();
           if (one) {
@@ -1076,7 +1120,7 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
               pend = true;
               pend_q = r;
               pend_p0 = r0;
-              pend_tf = tflag(tile, m);
+              pend_tf = tflag(tile, whole ? 1 : m);
               pend_near = in && near0(me0, r0, c);
               pend_cnt = 0;
             }
@@ -1764,101 +1808,84 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
   vel[s] = make_float4(vx, vy, vz, 0.0f);
 }
 
-// ---------------------------------------------------------------- repeated waves
+// ---------------------------------------------------------------- runs
 //
-// Consecutive sorted slots whose particles sit at one position with one
-// velocity (bit for bit, after gravity), in one cell, have the same
-// candidates and the same tests: their lists differ only in which slot
-// each leaves out as itself.  When every slot between two of them is such
-// a particle too, the two lists hold the same values in the same order, so
-// every pass computes bit-identical results for them.  A chain of
-// consecutive 64-slot waves that are each uniform in that sense and share
-// one key is therefore computed by its first wave alone; the others skip
-// every pass (their lanes are inactive) and copy the first wave's outputs
-// after each pass (k_dedup_fill).
+// Particles at one position with one velocity (bit for bit, after gravity),
+// in one cell, have the same candidates and the same tests: their lists
+// differ only in which slot each leaves out as itself.  When every slot
+// between two of them is such a particle too, the two lists hold the same
+// values in the same order, so every pass computes bit-identical results for
+// them.  A maximal run of such consecutive slots is therefore computed by
+// its first slot alone (one lane per run), and the others copy its outputs
+// after each pass (k_run_fill).
 
-// link[w] = 1 when wave w and wave w-1 are both uniform with the same key.
-__global__ __launch_bounds__(1024) void k_dedup_link(StepConsts c,
-                                                     const float4* __restrict__ posB,
-                                                     const float4* __restrict__ velB,
-                                                     const int* __restrict__ slot_key,
-                                                     int* __restrict__ link,
-                                                     const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
-  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int w = s0 >> 6;
-  const int lane = threadIdx.x & (kWave - 1);
-  // uniform: all 64 slots of the wave exist and equal its first slot
-  auto uniform = [&](int wave, float3& x0, float3& v0, int& k0) {
-    const int sl = wave * kWave + lane;
-    const bool in = sl < c.n;
-    const int t = in ? sl : c.n - 1;
-    const float3 x = ld3(posB, t);
-    const float3 v = ld3(velB, t);
-    const int k = slot_key[t];
-    x0 = bcast(x, 0);
-    v0 = bcast(v, 0);
-    k0 = bcast(k, 0);
-    return __all(in && same_bits(x, x0) && same_bits(v, v0) && k == k0);
-  };
-  float3 xa, va, xb, vb;
-  int ka, kb;
-  const bool ua = uniform(w, xa, va, ka);
-  bool l = false;
-  if (w > 0) {
-    const bool ub = uniform(w - 1, xb, vb, kb);
-    l = ua && ub && ka == kb && same_bits(xa, xb) && same_bits(va, vb);
-  }
-  const int nwaves = (c.n + kWave - 1) / kWave;
-  if (lane == 0 && w < nwaves) link[w] = l ? 1 : 0;
-}
-
-// head[w] = the first wave of w's chain: the last wave v <= w with
-// link[v] = 0 (one workgroup: a running max over each thread's span, then
-// a max-scan across threads).
-__global__ __launch_bounds__(1024) void k_dedup_heads(int nwaves, const int* __restrict__ link,
-                                                      int* __restrict__ head,
-                                                      const unsigned* __restrict__ flags) {
-  __shared__ int sh[1024];
-  if (fatal(flags)) return;
-  const int T = blockDim.x;
-  const int span = (nwaves + T - 1) / T;
-  const int a = threadIdx.x * span;
-  const int b = min(nwaves, a + span);
-  int run = -1;
-  for (int w = a; w < b; ++w)
-    if (!link[w]) run = w;
-  sh[threadIdx.x] = run;
-  __syncthreads();
-  for (int o = 1; o < T; o <<= 1) {
-    const int v = threadIdx.x >= static_cast<unsigned>(o) ? sh[threadIdx.x - o] : -1;
-    __syncthreads();
-    sh[threadIdx.x] = max(sh[threadIdx.x], v);
-    __syncthreads();
-  }
-  int cur = threadIdx.x > 0 ? sh[threadIdx.x - 1] : -1;
-  for (int w = a; w < b; ++w) {
-    if (!link[w]) cur = w;
-    head[w] = cur;
-  }
-}
-
-// Copies the chain's first wave's output into every repeated wave's slots.
-// With `ids`, w is each slot's own original id, ids[s].w (the committed
-// position carries the id of the slot's input position, which differs from
-// the first wave's).
-__global__ __launch_bounds__(1024) void k_dedup_fill(StepConsts c,
-                                                     const int* __restrict__ head,
-                                                     float4* __restrict__ a,
-                                                     const float4* __restrict__ ids,
-                                                     const unsigned* __restrict__ flags) {
+// flag[s] = 1 when slot s starts a run.
+__global__ __launch_bounds__(1024) void k_run_flags(StepConsts c, const float4* __restrict__ posB,
+                                                    const float4* __restrict__ velB,
+                                                    const int* __restrict__ slot_key,
+                                                    int* __restrict__ flag,
+                                                    const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
-  const int w = s >> 6;
-  const int h = head[w];
-  if (h == w) return;
-  float4 v = a[h * kWave];
+  bool head = s == 0;
+  if (!head)
+    head = slot_key[s] != slot_key[s - 1] || !same_bits(ld3(posB, s), ld3(posB, s - 1)) ||
+           !same_bits(ld3(velB, s), ld3(velB, s - 1));
+  flag[s] = head ? 1 : 0;
+}
+
+// From the scanned flags: the list of run heads, and each slot's run number
+// (in place of its flag).  Slots that are not heads store no neighbor list.
+__global__ __launch_bounds__(1024) void k_run_list(StepConsts c, int* __restrict__ run_no,
+                                                   const int* __restrict__ excl,
+                                                   int* __restrict__ rep,
+                                                   long long* __restrict__ noff,
+                                                   int* __restrict__ nwords,
+                                                   int* __restrict__ ncount,
+                                                   const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int head = run_no[s];
+  const int e = excl[s];
+  if (head) {
+    rep[e] = s;
+  } else {
+    noff[s] = 0;
+    nwords[s] = -1;
+    ncount[s] = 0;
+  }
+  run_no[s] = e + head - 1;
+}
+
+// run_end[s]: one past the last slot of s's run (the next run's first slot).
+__global__ __launch_bounds__(1024) void k_run_ends(StepConsts c, const int* __restrict__ run_no,
+                                                   const int* __restrict__ excl,
+                                                   const int* __restrict__ rep,
+                                                   int* __restrict__ run_end,
+                                                   const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int r = run_no[s] + 1;
+  run_end[s] = r < excl[c.n] ? rep[r] : c.n;
+}
+
+// Copies each run's first slot's output into the run's other slots.  With
+// `ids`, w is each slot's own original id, ids[s].w (the committed position
+// carries the id of the slot's input position).
+__global__ __launch_bounds__(1024) void k_run_fill(StepConsts c, const int* __restrict__ run_no,
+                                                   const int* __restrict__ rep,
+                                                   float4* __restrict__ a,
+                                                   const float4* __restrict__ ids,
+                                                   const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int h = rep[run_no[s]];
+  if (h == s) return;
+  float4 v = a[h];
   if (ids) v.w = ids[s].w;
   a[s] = v;
 }
@@ -1902,8 +1929,9 @@ hipError_t enqueue_count_pairs(Ctx& c) {
   if (c.n == 0) return hipGetLastError();
   StepConsts k = c.consts;
   k.count_only = 1;
-  k.dedup = 0;
-  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg, c.wave_head, c.pred[0]};
+  k.runs = 0;
+  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg, nullptr, nullptr, nullptr,
+                    c.pred[0]};
   k_neighbors<<<blocks_for(c.n, c.block), c.block, 0, c.stream>>>(
       k, c.pred[0], cv, c.noff, c.nwords, c.ncount, c.nbr, c.nbr_cap, c.pair_total, c.flags);
   return hipGetLastError();
@@ -1939,20 +1967,21 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   // recording's own array per iterate
   auto PK = [&](int k) { return rec ? rec->P[k] : iterate_buffer(c, k); };
   Launcher L{c};
-  const CellView cv{c.slot_cell, c.slot_key, c.grid, c.start, c.seg, c.wave_head, PK(0)};
   StepConsts k = c.consts;
   // a recorded substep keeps every particle's own list (the adjoint reads them)
   if (rec) {
-    k.dedup = 0;
+    k.runs = 0;
     k.full_lists = 1;
   }
+  const bool runs = k.runs != 0;
+  const CellView cv{c.slot_cell, c.slot_key, c.grid,     c.start,
+                    c.seg,       c.rep_list, c.run_excl + c.n, runs ? c.run_end : nullptr,
+                    PK(0)};
   c.lists_partial = !rec && k.coop && !k.full_lists;
-  const bool dedup = k.dedup != 0;
-  const int nwaves = (c.n + kWave - 1) / kWave;
   auto fill = [&](float4* a, const float4* ids) {
-    if (dedup)
-      k_dedup_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(k, c.wave_head, a, ids,
-                                                                c.flags);
+    if (runs)
+      k_run_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(k, c.run_no, c.rep_list, a, ids,
+                                                             c.flags);
   };
   const int bs = c.block;
   const int nb = blocks_for(c.n, bs);
@@ -1994,11 +2023,18 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                  c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
-  if (dedup)
-    FH_TRY(L("dedup_plan", [&] {
-      k_dedup_link<<<blocks_for(static_cast<long long>(nwaves) * kWave, 256), 256, 0, st>>>(
-          k, c.posB, c.velB, c.slot_key, c.wave_link, c.flags);
-      k_dedup_heads<<<1, 1024, 0, st>>>(nwaves, c.wave_link, c.wave_head, c.flags);
+  if (runs)
+    FH_TRY(L("runs", [&] {
+      k_run_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.posB, c.velB, c.slot_key, c.run_no,
+                                                         c.flags);
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.n_dev, c.scan_partial, c.flags);
+      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.n_dev, c.scan_partial, c.run_excl,
+                                                 0, c.flags);
+      k_run_list<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
+                                                        c.noff, c.nwords, c.ncount, c.flags);
+      k_run_ends<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
+                                                        c.run_end, c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nb, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
