@@ -224,7 +224,9 @@ void free_all(Ctx& c) {
   for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
                   &c.total_cells, &c.scan_partial, &c.slot_key, &c.nbr, &c.seg, &c.nwords,
                   &c.ncount, &c.tile_bbox, &c.run_no, &c.run_excl, &c.rep_list, &c.run_end,
-                  &c.n_dev,
+                  &c.n_dev, &c.cell_no, &c.cell_excl, &c.cell_first, &c.lanes1, &c.lanes2,
+                  &c.off1, &c.off2,
+                  &c.lane_rep, &c.lane_info,
                   &c.steps_started})
     dfree(*p);
   dfree(c.slot_cell);
@@ -618,6 +620,17 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.run_excl, static_cast<size_t>(n) + 1);
   ALLOC(c.rep_list, n);
   ALLOC(c.run_end, n);
+  ALLOC(c.cell_no, n);
+  ALLOC(c.cell_excl, static_cast<size_t>(n) + 1);
+  ALLOC(c.cell_first, static_cast<size_t>(n) + 1);
+  ALLOC(c.lanes1, static_cast<size_t>(n) + 1);
+  ALLOC(c.lanes2, static_cast<size_t>(n) + 1);
+  ALLOC(c.off1, static_cast<size_t>(n) + 1);
+  ALLOC(c.off2, static_cast<size_t>(n) + 1);
+  // every dense cell (>= 32 slots) pads its runs by at most 63 lanes
+  c.lane_cap = 3ll * n + 64;
+  ALLOC(c.lane_rep, static_cast<size_t>(c.lane_cap));
+  ALLOC(c.lane_info, 3);
   ALLOC(c.n_dev, 1);
   ALLOC(c.grid, static_cast<size_t>(n_scenes));
   ALLOC(c.total_cells, 1);

@@ -81,6 +81,18 @@ struct Ctx {
   int* rep_list = nullptr;   // per run: its first slot
   int* run_end = nullptr;    // per slot: one past the last slot of its run
   int* n_dev = nullptr;      // the particle count, on the device (scan length)
+  // lanes of the passes: runs of sparse cells packed, then each dense cell's
+  // runs from a wave boundary (one cell per wave)
+  int* cell_no = nullptr;    // per slot: cell head flag, then the slot's non-empty cell number
+  int* cell_excl = nullptr;  // n + 1: exclusive scan of the cell head flags; [n] = cells
+  int* cell_first = nullptr; // per non-empty cell: its first slot
+  int* lanes1 = nullptr;     // per non-empty cell: packed lanes
+  int* lanes2 = nullptr;     // per non-empty cell: aligned lanes (whole waves)
+  int* off1 = nullptr;       // n + 1: exclusive scan of lanes1; [cells] = total
+  int* off2 = nullptr;       // n + 1: exclusive scan of lanes2; [cells] = total
+  int* lane_rep = nullptr;   // lane_cap: the run head slot of each lane, -1 for padding
+  int* lane_info = nullptr;  // 3: packed lanes, first aligned lane, end of aligned lanes
+  long long lane_cap = 0;    // lanes the passes launch with runs: 3n + 64
 
   // neighbors
   long long* noff = nullptr;  // first word of each slot's list

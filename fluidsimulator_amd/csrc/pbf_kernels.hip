@@ -647,17 +647,19 @@ struct CellView {
   const GridDesc* __restrict__ grid;
   const int* __restrict__ start;
   const int* __restrict__ seg;
-  const int* __restrict__ rep;             // with runs: the first slot of each run
-  const int* __restrict__ n_reps;          // with runs: the number of runs
+  const int* __restrict__ lane_rep;        // with runs: each lane's run head slot (-1: none)
+  const int* __restrict__ lane_info;       // with runs: packed lanes, aligned lanes' range
   const int* __restrict__ run_end;         // with runs: per slot, one past its run's end
   const float4* __restrict__ p0;           // predicted positions p^0 (xyz) of the lists
 };
 
 // One lane's view: its slot (the last particle for lanes past the end, so
 // that every wavefront stays whole), cell and whether it walks
-// cooperatively ("dense").  With runs, lane i computes the first slot of
-// run i (the run's other slots copy its outputs) and lanes past the last
-// run are inactive.
+// cooperatively ("dense").  With runs, a lane computes the first slot of
+// one run (the run's other slots copy its outputs): the runs of sparse
+// cells are packed from lane 0, and each dense cell's runs start at a wave
+// boundary after them, so that a wave walks one dense cell only; padding
+// lanes are inactive.
 struct LaneCell {
   int s;
   bool active;
@@ -674,8 +676,11 @@ __device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellVie
                                               const int* __restrict__ nwords = nullptr) {
   LaneCell L;
   if (c.runs) {
-    L.active = s0 < *cv.n_reps;
-    L.s = L.active ? cv.rep[s0] : c.n - 1;
+    const int packed = cv.lane_info[0];
+    int slot = -1;
+    if (s0 < packed || (s0 >= cv.lane_info[1] && s0 < cv.lane_info[2])) slot = cv.lane_rep[s0];
+    L.active = slot >= 0;
+    L.s = L.active ? slot : c.n - 1;
   } else {
     L.active = s0 < c.n;
     L.s = L.active ? s0 : c.n - 1;
@@ -1872,6 +1877,94 @@ __global__ __launch_bounds__(1024) void k_run_ends(StepConsts c, const int* __re
   run_end[s] = r < excl[c.n] ? rep[r] : c.n;
 }
 
+// flag[s] = 1 when slot s starts a (non-empty) cell.
+__global__ __launch_bounds__(1024) void k_cell_flags(StepConsts c, const int* __restrict__ slot_key,
+                                                     int* __restrict__ flag,
+                                                     const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  flag[s] = (s == 0 || slot_key[s] != slot_key[s - 1]) ? 1 : 0;
+}
+
+// From the scanned flags: each non-empty cell's first slot, and each slot's
+// cell number (in place of its flag).
+__global__ __launch_bounds__(1024) void k_cell_list(StepConsts c, int* __restrict__ cell_no,
+                                                    const int* __restrict__ excl,
+                                                    int* __restrict__ first,
+                                                    const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int head = cell_no[s];
+  const int e = excl[s];
+  if (head) first[e] = s;
+  cell_no[s] = e + head - 1;
+}
+
+__device__ __forceinline__ bool aligned_cell(const StepConsts& c, int a, int b) {
+  return c.coop && (b - a) >= kDenseCell;
+}
+
+// Lanes of non-empty cell j: its runs, packed (lanes1) or rounded up to whole
+// waves (lanes2, dense cells).
+__global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __restrict__ excl,
+                                                     const int* __restrict__ first,
+                                                     const int* __restrict__ run_no,
+                                                     int* __restrict__ lanes1,
+                                                     int* __restrict__ lanes2,
+                                                     const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int C = excl[c.n];
+  if (j >= C) return;
+  const int a = first[j];
+  const int b = j + 1 < C ? first[j + 1] : c.n;
+  const int R = run_no[b - 1] - run_no[a] + 1;
+  const bool al = aligned_cell(c, a, b);
+  lanes1[j] = al ? 0 : R;
+  lanes2[j] = al ? (R + kWave - 1) / kWave * kWave : 0;
+}
+
+// lane_rep: each run's head slot at its lane (off1/off2: the cells' scanned
+// lane counts); padding lanes of dense cells get -1.  lane_info = {packed
+// lanes, first aligned lane, end of aligned lanes}.
+__global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __restrict__ run_no,
+                                                   const int* __restrict__ cell_no,
+                                                   const int* __restrict__ first,
+                                                   const int* __restrict__ excl,
+                                                   const int* __restrict__ off1,
+                                                   const int* __restrict__ off2,
+                                                   int* __restrict__ lane_rep,
+                                                   int* __restrict__ lane_info,
+                                                   const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int C = excl[c.n];
+  const int total1 = off1[C];
+  const int base2 = (total1 + kWave - 1) / kWave * kWave;
+  if (s == 0) {
+    lane_info[0] = total1;
+    lane_info[1] = base2;
+    lane_info[2] = base2 + off2[C];
+  }
+  if (s > 0 && run_no[s] == run_no[s - 1]) return;  // not a run head
+  const int j = cell_no[s];
+  const int a = first[j];
+  const int b = j + 1 < C ? first[j + 1] : c.n;
+  const int rr = run_no[s] - run_no[a];
+  if (!aligned_cell(c, a, b)) {
+    lane_rep[off1[j] + rr] = s;
+    return;
+  }
+  const int lane0 = base2 + off2[j];
+  lane_rep[lane0 + rr] = s;
+  const int R = run_no[b - 1] - run_no[a] + 1;
+  if (rr == R - 1)
+    for (int q = R; q < (R + kWave - 1) / kWave * kWave; ++q) lane_rep[lane0 + q] = -1;
+}
+
 // Copies each run's first slot's output into the run's other slots.  With
 // `ids`, w is each slot's own original id, ids[s].w (the committed position
 // carries the id of the slot's input position).
@@ -1974,8 +2067,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     k.full_lists = 1;
   }
   const bool runs = k.runs != 0;
-  const CellView cv{c.slot_cell, c.slot_key, c.grid,     c.start,
-                    c.seg,       c.rep_list, c.run_excl + c.n, runs ? c.run_end : nullptr,
+  const CellView cv{c.slot_cell, c.slot_key, c.grid,      c.start,
+                    c.seg,       c.lane_rep, c.lane_info, runs ? c.run_end : nullptr,
                     PK(0)};
   c.lists_partial = !rec && k.coop && !k.full_lists;
   auto fill = [&](float4* a, const float4* ids) {
@@ -1985,6 +2078,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   };
   const int bs = c.block;
   const int nb = blocks_for(c.n, bs);
+  // neighbor build and passes: one lane per slot, or with runs the lane map
+  const int nl = runs ? blocks_for(c.lane_cap, bs) : nb;
   hipStream_t st = c.stream;
   hipError_t e;
 
@@ -2035,9 +2130,31 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                                         c.noff, c.nwords, c.ncount, c.flags);
       k_run_ends<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
                                                         c.run_end, c.flags);
+      // lanes: non-empty cells, their lane counts, offsets, the lane map
+      k_cell_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.slot_key, c.cell_no, c.flags);
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.cell_no, c.n_dev, c.scan_partial, c.flags);
+      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.cell_no, c.n_dev, c.scan_partial,
+                                                 c.cell_excl, 0, c.flags);
+      k_cell_list<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_no, c.cell_excl, c.cell_first,
+                                                         c.flags);
+      k_cell_lanes<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_excl, c.cell_first, c.run_no,
+                                                          c.lanes1, c.lanes2, c.flags);
+      for (int q = 0; q < 2; ++q) {
+        int* lanes = q == 0 ? c.lanes1 : c.lanes2;
+        int* off = q == 0 ? c.off1 : c.off2;
+        k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(lanes, c.cell_excl + c.n, c.scan_partial,
+                                                     c.flags);
+        k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
+        k_scan_down<<<c.scan_blocks, 256, 0, st>>>(lanes, c.cell_excl + c.n, c.scan_partial, off,
+                                                   0, c.flags);
+      }
+      k_lane_map<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.cell_no, c.cell_first,
+                                                        c.cell_excl, c.off1, c.off2, c.lane_rep,
+                                                        c.lane_info, c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
-    k_neighbors<<<nb, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
+    k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
   }));
 
@@ -2055,20 +2172,20 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     unsigned* pm = rec ? rec->M[it] : nullptr;
     unsigned* rm = rec ? rec->RM : nullptr;
     FH_TRY(L("lambda", [&] {
-      k_lambda<<<nb, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
+      k_lambda<<<nl, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
       fill(P, nullptr);
     }));
     FH_TRY(L("delta", [&] {
       if (!last)
-        k_delta<true, false, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
+        k_delta<true, false, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
                                                        pm, nullptr, c.flags);
       else if (rest_in_delta)
-        k_delta<true, true, true><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
+        k_delta<true, true, true><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                      c.planes, c.posB, c.posA, v1, c.rho,
                                                      pm, rm, c.flags);
       else
-        k_delta<true, true, false><<<nb, bs, 0, st>>>(k, P, Pn, nv, cv,
+        k_delta<true, true, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
                                                       pm, nullptr, c.flags);
       if (!last) {
@@ -2082,11 +2199,11 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (I == 0) {
     FH_TRY(L("delta", [&] {
       if (rest_in_delta)
-        k_delta<false, true, true><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
+        k_delta<false, true, true><<<nl, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
                                                       c.planes, c.posB, c.posA, v1, c.rho,
                                                       nullptr, nullptr, c.flags);
       else
-        k_delta<false, true, false><<<nb, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
+        k_delta<false, true, false><<<nl, bs, 0, st>>>(k, c.pred[0], c.pred[1], nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
                                                        nullptr, nullptr, c.flags);
       fill(c.posA, c.posB);
@@ -2097,25 +2214,25 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     const bool rest = c.restitution_any && !c.vort;
     FH_TRY(L("xsph", [&] {
       if (rest)
-        k_xsph<true><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
+        k_xsph<true><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                         c.planes, rec ? rec->RM : nullptr, c.flags);
       else
-        k_xsph<false><<<nb, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
+        k_xsph<false><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                          c.planes, nullptr, c.flags);
       fill(c.velA, nullptr);
     }));
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
-      k_vort_omega<<<nb, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
+      k_vort_omega<<<nl, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
       fill(c.omega, nullptr);
     }));
     FH_TRY(L("vorticity_apply", [&] {
       if (c.restitution_any)
-        k_vort_apply<true><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
+        k_vort_apply<true><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                               c.planes, c.flags);
       else
-        k_vort_apply<false><<<nb, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
+        k_vort_apply<false><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                                c.planes, c.flags);
       fill(c.velA, nullptr);
     }));
