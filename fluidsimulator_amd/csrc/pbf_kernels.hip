@@ -505,7 +505,10 @@ __global__ __launch_bounds__(1024) void k_reorder(
     float4* __restrict__ pred, const uint64_t* __restrict__ cellc,
     uint64_t* __restrict__ slot_cell, int* __restrict__ slot_key,
     const unsigned* __restrict__ flags) {
-  const int u0 = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  // plain workgroup order: the waves of one dense cell, which all read that
+  // cell's id tiles, then spread over every XCD's L2 (measured 3.5 ms
+  // against 6.1 ms with the XCD order on the piled-up million-particle block)
+  const int u0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (fatal(flags)) return;
   // lanes past the end stand in for the last particle so that the
   // wavefront stays whole for the cooperative ranks below
