@@ -1910,10 +1910,16 @@ __device__ __forceinline__ bool aligned_cell(const StepConsts& c, int a, int b) 
 }
 
 // Lanes of non-empty cell j: its runs, packed (lanes1) or rounded up to whole
-// waves (lanes2, dense cells).
+// waves (lanes2): dense cells, and (up to 2^24 particles, which keeps the
+// padded total within int) sparse cells whose 27-cell neighborhood holds at
+// least kHeavyList candidates, whose particles walk cooperatively too.
 __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __restrict__ excl,
                                                      const int* __restrict__ first,
                                                      const int* __restrict__ run_no,
+                                                     const uint64_t* __restrict__ slot_cell,
+                                                     const GridDesc* __restrict__ grid,
+                                                     const int* __restrict__ start,
+                                                     const int* __restrict__ seg,
                                                      int* __restrict__ lanes1,
                                                      int* __restrict__ lanes2,
                                                      const unsigned* __restrict__ flags) {
@@ -1924,7 +1930,25 @@ __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __
   const int a = first[j];
   const int b = j + 1 < C ? first[j + 1] : c.n;
   const int R = run_no[b - 1] - run_no[a] + 1;
-  const bool al = aligned_cell(c, a, b);
+  bool al = aligned_cell(c, a, b);
+  if (!al && c.coop && c.n <= (1 << 24)) {
+    int x, y, z;
+    unpack_cell(slot_cell[a], x, y, z);
+    const GridDesc g = grid[scene_of(a, seg, c.n_scenes)];
+    const int rx = x - g.ox, ry = y - g.oy, rz = z - g.oz;
+    const int ext = 1 << g.bshift;
+    if (rx >= 1 && ry >= 1 && rz >= 1 && rx <= ext * g.nbx - 2 && ry <= ext * g.nby - 2 &&
+        rz <= ext * g.nbz - 2) {
+      int cand = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int k = cell_key(rx + dx, ry + dy, rz + dz, g);
+            cand += start[k + 1] - start[k];
+          }
+      al = cand >= kHeavyList;
+    }
+  }
   lanes1[j] = al ? 0 : R;
   lanes2[j] = al ? (R + kWave - 1) / kWave * kWave : 0;
 }
@@ -1936,8 +1960,11 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
                                                    const int* __restrict__ cell_no,
                                                    const int* __restrict__ first,
                                                    const int* __restrict__ excl,
+                                                   const int* __restrict__ lanes2,
                                                    const int* __restrict__ off1,
                                                    const int* __restrict__ off2,
+                                                   const int* __restrict__ runs_total,
+                                                   long long lane_cap,
                                                    int* __restrict__ lane_rep,
                                                    int* __restrict__ lane_info,
                                                    const unsigned* __restrict__ flags) {
@@ -1947,17 +1974,25 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
   const int C = excl[c.n];
   const int total1 = off1[C];
   const int base2 = (total1 + kWave - 1) / kWave * kWave;
+  const long long end2 = static_cast<long long>(base2) + off2[C];
+  // beyond the lane capacity: every run at its own number, packed
+  const bool fits = end2 <= lane_cap;
   if (s == 0) {
-    lane_info[0] = total1;
-    lane_info[1] = base2;
-    lane_info[2] = base2 + off2[C];
+    const int R = *runs_total;
+    lane_info[0] = fits ? total1 : R;
+    lane_info[1] = fits ? base2 : R;
+    lane_info[2] = fits ? static_cast<int>(end2) : R;
   }
   if (s > 0 && run_no[s] == run_no[s - 1]) return;  // not a run head
+  if (!fits) {
+    lane_rep[run_no[s]] = s;
+    return;
+  }
   const int j = cell_no[s];
   const int a = first[j];
   const int b = j + 1 < C ? first[j + 1] : c.n;
   const int rr = run_no[s] - run_no[a];
-  if (!aligned_cell(c, a, b)) {
+  if (lanes2[j] == 0) {
     lane_rep[off1[j] + rr] = s;
     return;
   }
@@ -2142,6 +2177,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
       k_cell_list<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_no, c.cell_excl, c.cell_first,
                                                          c.flags);
       k_cell_lanes<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_excl, c.cell_first, c.run_no,
+                                                          c.slot_cell, c.grid, c.start, c.seg,
                                                           c.lanes1, c.lanes2, c.flags);
       for (int q = 0; q < 2; ++q) {
         int* lanes = q == 0 ? c.lanes1 : c.lanes2;
@@ -2152,9 +2188,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         k_scan_down<<<c.scan_blocks, 256, 0, st>>>(lanes, c.cell_excl + c.n, c.scan_partial, off,
                                                    0, c.flags);
       }
-      k_lane_map<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.cell_no, c.cell_first,
-                                                        c.cell_excl, c.off1, c.off2, c.lane_rep,
-                                                        c.lane_info, c.flags);
+      k_lane_map<<<blocks_for(c.n, 256), 256, 0, st>>>(
+          k, c.run_no, c.cell_no, c.cell_first, c.cell_excl, c.lanes2, c.off1, c.off2,
+          c.run_excl + c.n, c.lane_cap, c.lane_rep, c.lane_info, c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
