@@ -40,7 +40,7 @@ KERNEL_BYTES = {
     "vorticity_apply": 52.0,   # pos 12 + omega 12 + |omega| 4 + vel 12 -> vel 12
 }
 STEP_BYTES_ALL_FLAGS = 480.0   # whole substep, all flags, 4 iterations
-KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "scatter", "reorder",
+KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "runs",
            "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply"]
 
 

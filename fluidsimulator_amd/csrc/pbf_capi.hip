@@ -221,7 +221,8 @@ void free_all(Ctx& c) {
     dfree(*p);
   dfree(c.rho);
   dfree(c.cellc);
-  for (int** p : {&c.key, &c.rank, &c.perm, &c.sid, &c.skey, &c.count, &c.start, &c.bbox,
+  for (int** p : {&c.key, &c.sort_k[0], &c.sort_k[1], &c.sort_v[0], &c.sort_v[1],
+                  &c.radix_hist, &c.radix_off, &c.radix_len, &c.count, &c.start, &c.bbox,
                   &c.total_cells, &c.scan_partial, &c.slot_key, &c.nbr, &c.seg, &c.nwords,
                   &c.ncount, &c.tile_bbox, &c.run_no, &c.run_excl, &c.rep_list, &c.run_end,
                   &c.n_dev, &c.cell_no, &c.cell_excl, &c.cell_first, &c.lanes1, &c.lanes2,
@@ -605,10 +606,14 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.rho, n);
   ALLOC(c.cellc, n);
   ALLOC(c.key, n);
-  ALLOC(c.rank, n);
-  ALLOC(c.perm, n);
-  ALLOC(c.sid, n);
-  ALLOC(c.skey, n);
+  for (int q = 0; q < 2; ++q) {
+    ALLOC(c.sort_k[q], n);
+    ALLOC(c.sort_v[q], n);
+  }
+  c.radix_blocks = (n + fluidhip::kRadixTile - 1) / fluidhip::kRadixTile;
+  ALLOC(c.radix_hist, 256 * static_cast<size_t>(c.radix_blocks));
+  ALLOC(c.radix_off, 256 * static_cast<size_t>(c.radix_blocks) + 1);
+  ALLOC(c.radix_len, 1);
   ALLOC(c.slot_cell, n);
   ALLOC(c.slot_key, n);
   ALLOC(c.noff, n);
@@ -646,6 +651,11 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
 #undef ALLOC
   e = hipMemcpy(c.n_dev, &c.n, sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(particle count)"));
+  {
+    const int len = 256 * c.radix_blocks;
+    e = hipMemcpy(c.radix_len, &len, sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(radix length)"));
+  }
   // Defaults leave room for the spray and pile-up of violent scenes (cell
   // tables of tens of millions of cells, run-length lists well above the
   // particle count); both grow on demand.

@@ -60,11 +60,15 @@ struct Ctx {
 
   // grid
   uint64_t* cellc = nullptr;  // packed cell of pred0, previous order
-  int* key = nullptr;
-  int* rank = nullptr;
-  int* perm = nullptr;
-  int* sid = nullptr;
-  int* skey = nullptr;
+  int* key = nullptr;  // per slot of the previous order: cell key
+  // stable radix sort of (cell key) over particles in id order; values are
+  // previous slots, so the sorted values list the new (cell, id) order
+  int* sort_k[2] = {nullptr, nullptr};
+  int* sort_v[2] = {nullptr, nullptr};
+  int* radix_hist = nullptr;  // 256 digits x radix_blocks
+  int* radix_off = nullptr;   // its exclusive scan (+1)
+  int* radix_len = nullptr;   // 256 * radix_blocks, on the device (scan length)
+  int radix_blocks = 0;
   int* count = nullptr;  // cell_cap
   int* start = nullptr;  // cell_cap + 1
   long long cell_cap = 0;

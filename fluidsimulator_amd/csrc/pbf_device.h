@@ -22,6 +22,9 @@ constexpr unsigned kFlagFatal = kFlagRange | kFlagCells | kFlagNeighbors;
 
 // Loop-invariant constants of one substep, computed on the host with the
 // same float expressions the reference evaluates inside its loops.
+// Items per workgroup of the radix sort that orders slots by (cell, id).
+constexpr int kRadixTile = 4096;
+
 struct StepConsts {
   int n;            // particles, all scenes
   int n_scenes;

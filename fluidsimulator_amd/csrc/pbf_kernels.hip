@@ -391,7 +391,6 @@ __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
                                                      const int* __restrict__ seg,
                                                      const GridDesc* __restrict__ grid,
                                                      int* __restrict__ count, int* __restrict__ key,
-                                                     int* __restrict__ rank,
                                                      const unsigned* __restrict__ flags) {
   const int i0 = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
   if (fatal(flags)) return;
@@ -401,21 +400,15 @@ __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
   int x, y, z;
   unpack_cell(cellc[i], x, y, z);
   const int k = cell_key(x - g.ox, y - g.oy, z - g.oz, g);
-  // Counting sort, wavefront-aggregated: the particles arrive in the
+  // Cell histogram, wavefront-aggregated: the particles arrive in the
   // previous substep's sorted order, so neighboring lanes mostly share a
-  // cell.  Each group of lanes with one key takes one atomic for all of
-  // them and ranks itself by a ballot prefix.
+  // cell, and each group of lanes with one key takes one atomic.
   const int lane = threadIdx.x & (kWave - 1);
-  int r = 0;
   for_each_key_group(active, k, [&](int leader, bool mine) {
     const unsigned long long gm = __ballot(mine);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count + k, __popcll(gm));
-    base = __builtin_amdgcn_readlane(base, leader);
-    r = mine ? base + __popcll(gm & ((1ull << lane) - 1ull)) : r;
+    if (lane == leader) atomicAdd(count + k, __popcll(gm));
   });
   if (!active) return;
-  rank[i] = r;
   key[i] = k;
 }
 
@@ -479,82 +472,120 @@ __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
   if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = partial[G];
 }
 
-__global__ __launch_bounds__(1024) void k_scatter(StepConsts c, const int* __restrict__ key,
-                                                  const int* __restrict__ rank,
-                                                  const int* __restrict__ start,
-                                                  const float4* __restrict__ pos,
-                                                  int* __restrict__ perm, int* __restrict__ sid,
-                                                  int* __restrict__ skey,
-                                                  const unsigned* __restrict__ flags) {
-  const int p = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (p >= c.n || fatal(flags)) return;
-  const int k = key[p];
-  const int slot = start[k] + rank[p];
-  perm[slot] = p;
-  sid[slot] = __float_as_int(reinterpret_cast<const float*>(pos + p)[3]);
-  skey[slot] = k;
+// ---- stable radix sort of the slots by (cell key, original id)
+//
+// The keys are written in original-id order, so a stable sort by key alone
+// leaves the particles of each cell in ascending id: the reference's order.
+// Each pass sorts 8 bits: a per-workgroup digit histogram, the device scan
+// of the histogram in digit-major order, and a stable scatter.
+
+// Keys and values in original-id order: k[id] = cell key, v[id] = slot.
+__global__ __launch_bounds__(1024) void k_sort_keys(StepConsts c, const float4* __restrict__ pos,
+                                                    const int* __restrict__ key,
+                                                    int* __restrict__ kout, int* __restrict__ vout,
+                                                    const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.n) return;
+  const int id = __float_as_int(reinterpret_cast<const float*>(pos + p)[3]);
+  if (id < 0 || id >= c.n) return;  // ids are 0..n-1
+  kout[id] = key[p];
+  vout[id] = p;
 }
 
-// Rank by original id inside the cell, then gather the particle into its
-// final slot and recompute pred0 there (same float ops as predict_bbox).
-__global__ __launch_bounds__(1024) void k_reorder(
-    StepConsts c, const int* __restrict__ perm, const int* __restrict__ sid,
-    const int* __restrict__ skey, const int* __restrict__ start, const int* __restrict__ seg,
-    const ScenePlanes* __restrict__ sp, const float4* __restrict__ posA,
-    const float4* __restrict__ velA, float4* __restrict__ posB, float4* __restrict__ velB,
-    float4* __restrict__ pred, const uint64_t* __restrict__ cellc,
-    uint64_t* __restrict__ slot_cell, int* __restrict__ slot_key,
-    const unsigned* __restrict__ flags) {
-  // plain workgroup order: the waves of one dense cell, which all read that
-  // cell's id tiles, then spread over every XCD's L2 (measured 3.5 ms
-  // against 6.1 ms with the XCD order on the piled-up million-particle block)
-  const int u0 = blockIdx.x * blockDim.x + threadIdx.x;
+// hist[d * G + b]: items of workgroup b (kRadixTile of them) with digit d.
+__global__ __launch_bounds__(256) void k_radix_hist(int n, const int* __restrict__ kin, int shift,
+                                                    int* __restrict__ hist,
+                                                    const unsigned* __restrict__ flags) {
+  __shared__ int h[256];
   if (fatal(flags)) return;
-  // lanes past the end stand in for the last particle so that the
-  // wavefront stays whole for the cooperative ranks below
-  const bool active = u0 < c.n;
-  const int u = active ? u0 : c.n - 1;
-  const int p = perm[u];
-  const int id = sid[u];
-  const int k = skey[u];
-  const int a = start[k];
-  const int b = start[k + 1];
-  int r = 0;
-  const bool dense = c.coop && (b - a) >= kDenseCell;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (c.coop) {
-    for_each_key_group(dense, k, [&](int leader, bool mine) {
-      // the group's cell: shared, coalesced id tiles
-      const int au = __builtin_amdgcn_readlane(a, leader);
-      const int bu = __builtin_amdgcn_readlane(b, leader);
-      for (int base = au; base < bu; base += kWave) {
-        const int m = min(kWave, bu - base);
-        const int tile = sid[base + min(lane, m - 1)];
-        // broadcasts run on the whole wavefront (a load whose only use sat
-        // under `mine` could be moved there, leaving other lanes' values
-        // unloaded); only the count is predicated
-        for (int j = 0; j < m; ++j) {
-          const int tj = bcast(tile, j);
-          r += (mine && tj < id) ? 1 : 0;
-        }
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int b0 = blockIdx.x * kRadixTile;
+  const int b1 = min(n, b0 + kRadixTile);
+  for (int i = b0 + static_cast<int>(threadIdx.x); i < b1; i += 256)
+    atomicAdd(&h[(kin[i] >> shift) & 255], 1);
+  __syncthreads();
+  hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one pass: the workgroup's items in index order, 256 per
+// round; an item goes to its digit's offset for this workgroup, plus the
+// items of that digit in earlier rounds, earlier wavefronts and lower lanes.
+__global__ __launch_bounds__(256) void k_radix_scatter(int n, const int* __restrict__ kin,
+                                                       const int* __restrict__ vin,
+                                                       int* __restrict__ kout,
+                                                       int* __restrict__ vout, int shift,
+                                                       const int* __restrict__ off,
+                                                       const unsigned* __restrict__ flags) {
+  __shared__ int run[256];      // per digit: where the next item of this workgroup goes
+  __shared__ int wcnt[4][256];  // per wavefront of the round: items per digit
+  if (fatal(flags)) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid >> 6;
+  run[tid] = off[tid * gridDim.x + blockIdx.x];
+  const int b0 = blockIdx.x * kRadixTile;
+  for (int r = 0; r < kRadixTile / 256; ++r) {
+    const int i = b0 + r * 256 + tid;
+    const bool ok = i < n;
+    const int kk = ok ? kin[i] : 0;
+    const int d = (kk >> shift) & 255;
+    // the lanes of this wavefront with the same digit
+    unsigned long long peers = __ballot(ok);
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool x = (d >> bit) & 1;
+      const unsigned long long bb = __ballot(x);
+      peers &= x ? bb : ~bb;
+    }
+    if (!ok) peers = 0ull;
+    const int below = __popcll(peers & ((1ull << lane) - 1ull));
+    for (int v = 0; v < 4; ++v) wcnt[v][tid] = 0;
+    __syncthreads();
+    if (ok && below == 0) wcnt[w][d] = __popcll(peers);
+    __syncthreads();
+    if (ok) {
+      int pos = run[d] + below;
+      for (int v = 0; v < w; ++v) pos += wcnt[v][d];
+      if (pos < n) {  // always, as the histogram counted these same items
+        kout[pos] = kk;
+        vout[pos] = vin[i];
       }
-    });
+    }
+    __syncthreads();
+    run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    // the next round's zeroing of wcnt[.][tid] follows this read in the same thread
   }
-  if (!dense)
-    for (int t = a; t < b; ++t) r += (sid[t] < id) ? 1 : 0;
-  const int dest = a + r;
-  const ScenePlanes& s = sp[scene_of(u, seg, c.n_scenes)];
+}
+
+// Gathers each particle into its slot of the (cell, id) order (sv: the
+// sorted previous slots, sk: their cell keys) and recomputes pred there.
+__global__ __launch_bounds__(1024) void k_reorder(
+    StepConsts c, const int* __restrict__ sk, const int* __restrict__ sv,
+    const int* __restrict__ seg, const ScenePlanes* __restrict__ sp,
+    const float4* __restrict__ posA, const float4* __restrict__ velA, float4* __restrict__ posB,
+    float4* __restrict__ velB, float4* __restrict__ pred, const uint64_t* __restrict__ cellc,
+    uint64_t* __restrict__ slot_cell, int* __restrict__ slot_key,
+    unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;
+  const int dest = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dest >= c.n) return;
+  const int p = sv[dest];
+  if (p < 0 || p >= c.n) {  // never for a complete sort; refuse rather than read outside
+    atomicOr(flags, kFlagRange);
+    return;
+  }
+  const ScenePlanes& s = sp[scene_of(dest, seg, c.n_scenes)];
   const float4 x = posA[p];
   const float4 v = velA[p];
   const float vx = v.x + s.gx * c.dt;
   const float vy = v.y + s.gy * c.dt;
   const float vz = v.z + s.gz * c.dt;
-  if (!active) return;
   posB[dest] = x;
   velB[dest] = make_float4(vx, vy, vz, 0.0f);
   pred[dest] = make_float4(x.x + vx * c.dt, x.y + vy * c.dt, x.z + vz * c.dt, 0.0f);
   slot_cell[dest] = cellc[p];
-  slot_key[dest] = k;
+  slot_key[dest] = sk[dest];
 }
 
 // ---------------------------------------------------------------- neighbors
@@ -2137,8 +2168,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                     c.flags, c.nbr_total, c.need_cells, c.steps_started);
   }));
   FH_TRY(L("cell_count", [&] {
-    k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.rank,
-                                    c.flags);
+    k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.flags);
   }));
   FH_TRY(L("scan", [&] {
     k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
@@ -2147,13 +2177,29 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
                                                c.start, 1, c.flags);
   }));
-  FH_TRY(L("scatter", [&] {
-    k_scatter<<<nb, bs, 0, st>>>(k, c.key, c.rank, c.start, c.posA, c.perm, c.sid, c.skey,
-                                 c.flags);
+  // keys lie below the cell table's capacity: 8 bits per pass
+  int key_bits = 1;
+  while (key_bits < 31 && (1ll << key_bits) < c.cell_cap) ++key_bits;
+  const int passes = (key_bits + 7) / 8;
+  FH_TRY(L("sort", [&] {
+    k_sort_keys<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.posA, c.key, c.sort_k[0],
+                                                       c.sort_v[0], c.flags);
+    for (int q = 0; q < passes; ++q) {
+      const int i = q & 1, o = i ^ 1;
+      k_radix_hist<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], 8 * q, c.radix_hist,
+                                                   c.flags);
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, c.radix_len, c.scan_partial,
+                                                   c.flags);
+      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, c.radix_len, c.scan_partial,
+                                                 c.radix_off, 0, c.flags);
+      k_radix_scatter<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], c.sort_v[i], c.sort_k[o],
+                                                      c.sort_v[o], 8 * q, c.radix_off, c.flags);
+    }
   }));
   FH_TRY(L("reorder", [&] {
-    k_reorder<<<nb, bs, 0, st>>>(k, c.perm, c.sid, c.skey, c.start, c.seg, c.planes, c.posA,
-                                 c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
+    k_reorder<<<nb, bs, 0, st>>>(k, c.sort_k[passes & 1], c.sort_v[passes & 1], c.seg, c.planes,
+                                 c.posA, c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
   if (runs)
