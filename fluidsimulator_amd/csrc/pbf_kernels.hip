@@ -690,10 +690,10 @@ struct CellView {
 // One lane's view: its slot (the last particle for lanes past the end, so
 // that every wavefront stays whole), cell and whether it walks
 // cooperatively ("dense").  With runs, a lane computes the first slot of
-// one run (the run's other slots copy its outputs): the runs of sparse
-// cells are packed from lane 0, and each dense cell's runs start at a wave
-// boundary after them, so that a wave walks one dense cell only; padding
-// lanes are inactive.
+// one run (the run's other slots copy its outputs): each dense (or heavy)
+// cell's runs start at a wave boundary, from lane 0, so that a wave walks
+// one such cell only and those long waves are dispatched first; the runs of
+// the other cells are packed after them; padding lanes are inactive.
 struct LaneCell {
   int s;
   bool active;
@@ -1985,8 +1985,8 @@ __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __
 }
 
 // lane_rep: each run's head slot at its lane (off1/off2: the cells' scanned
-// lane counts); padding lanes of dense cells get -1.  lane_info = {packed
-// lanes, first aligned lane, end of aligned lanes}.
+// lane counts); padding lanes of aligned cells get -1.  The lanes in use are
+// [0, lane_info[0]) and [lane_info[1], lane_info[2]).
 __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __restrict__ run_no,
                                                    const int* __restrict__ cell_no,
                                                    const int* __restrict__ first,
@@ -2003,16 +2003,16 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
   const int C = excl[c.n];
-  const int total1 = off1[C];
-  const int base2 = (total1 + kWave - 1) / kWave * kWave;
-  const long long end2 = static_cast<long long>(base2) + off2[C];
+  // dense and heavy cells first (their waves start first), packed runs after
+  const int total2 = off2[C];  // a multiple of 64
+  const long long end = static_cast<long long>(total2) + off1[C];
   // beyond the lane capacity: every run at its own number, packed
-  const bool fits = end2 <= lane_cap;
+  const bool fits = end <= lane_cap;
   if (s == 0) {
     const int R = *runs_total;
-    lane_info[0] = fits ? total1 : R;
-    lane_info[1] = fits ? base2 : R;
-    lane_info[2] = fits ? static_cast<int>(end2) : R;
+    lane_info[0] = fits ? total2 : R;
+    lane_info[1] = fits ? total2 : R;
+    lane_info[2] = fits ? static_cast<int>(end) : R;
   }
   if (s > 0 && run_no[s] == run_no[s - 1]) return;  // not a run head
   if (!fits) {
@@ -2024,10 +2024,10 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
   const int b = j + 1 < C ? first[j + 1] : c.n;
   const int rr = run_no[s] - run_no[a];
   if (lanes2[j] == 0) {
-    lane_rep[off1[j] + rr] = s;
+    lane_rep[total2 + off1[j] + rr] = s;
     return;
   }
-  const int lane0 = base2 + off2[j];
+  const int lane0 = off2[j];
   lane_rep[lane0 + rr] = s;
   const int R = run_no[b - 1] - run_no[a] + 1;
   if (rr == R - 1)
