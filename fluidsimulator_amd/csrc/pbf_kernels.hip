@@ -755,41 +755,17 @@ __device__ __forceinline__ Group group_of(const StepConsts& c, const CellView& c
   return gr;
 }
 
-// Calls body(base, m, tile) for each run of up to 64 candidates of the
-// group's 27-cell neighborhood, in walk order; tile holds load(candidate)
-// of lane `lane` (lanes past m repeat the last candidate).
-template <typename Load, typename Body>
-__device__ __forceinline__ void group_tiles(const Group& gr, const int* __restrict__ start,
-                                            Load&& load, Body&& body) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int k = __builtin_amdgcn_readfirstlane(
-            cell_key(gr.rx + dx, gr.ry + dy, gr.rz + dz, gr.g));
-        const int a = __builtin_amdgcn_readfirstlane(start[k]);
-        const int b = __builtin_amdgcn_readfirstlane(start[k + 1]);
-        if (a >= b) continue;
-        // the next tile's load is issued before this tile is processed
-        auto next = load(a + min(lane, min(kWave, b - a) - 1));
-        for (int base = a; base < b; base += kWave) {
-          const int m = min(kWave, b - base);
-          const auto tile = next;
-          const int nb = base + kWave;
-          if (nb < b) next = load(nb + min(lane, min(kWave, b - nb) - 1));
-          body(base, m, tile);
-        }
-      }
-}
 
 // Runs of identical slots at least this long are walked as one step.
 constexpr int kRunStep = 32;
 
-// Like group_tiles, but with run_end (runs on) a run of at least kRunStep
-// slots is one step: body(base, m, tile, true) with m the run's length and
-// lane 0 of `tile` holding slot base (every slot of the run holds the same
-// record).  Otherwise body(base, m, tile, false) for tiles of up to 64
-// candidates.  The next step's load is issued before a step is processed.
+// Walks the group's 27-cell neighborhood in (dz, dy, dx) order, each cell's
+// slots in ascending order.  Each step is body(base, m, tile, whole): with
+// run_end (runs on) a run of at least kRunStep slots is one step (whole,
+// m = its length, lane 0 of `tile` holding slot base: every slot of the run
+// holds the same record); otherwise a tile of up to 64 candidates, `tile`
+// holding load(candidate) of lane `lane` (lanes past m repeat the last).
+// The next step's load is issued before a step is processed.
 template <typename Load, typename Body>
 __device__ __forceinline__ void group_steps(const Group& gr, const int* __restrict__ start,
                                             const int* __restrict__ run_end, Load&& load,
@@ -937,12 +913,25 @@ __device__ __forceinline__ void group_walk_cells(const Group& gr, bool mine, int
                                                 float px, float py, float pz,
                                                 const StepConsts& c,
                                                 const int* __restrict__ start,
+                                                const int* __restrict__ run_end,
                                                 const float4* __restrict__ pred, RunWriter& w) {
   const float3 me = make_float3(px, py, pz);
   const bool on = c.coop_bulk != 0;
   const LaneClasses lc = lane_classes(me, mine, on);
-  group_tiles(gr, start, [&](int t) { return ld3(pred, t); },
-              [&](int base, int m, const float3& tile) {
+  group_steps(gr, start, run_end, [&](int t) { return ld3(pred, t); },
+              [&](int base, int m, const float3& tile, bool whole) {
+                if (whole) {
+                  // a run: every slot at one p^0, all taken or none
+                  if (mine && near0(me, bcast(tile, 0), c)) {
+                    if (s >= base && s < base + m) {
+                      w.add_range<kFill>(base, s);
+                      w.add_range<kFill>(s + 1, base + m);
+                    } else {
+                      w.add_range<kFill>(base, base + m);
+                    }
+                  }
+                  return;
+                }
                 unsigned long long any;
                 unsigned long long mask = tile_members(lc, me, mine, tile, m, s - base, on, c, any);
                 // the mask's runs of set bits, in order
@@ -1010,7 +999,7 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   if (c.coop)
     for_each_key_group(dense && listed, L.key, [&](int leader, bool mine) {
       group_walk_cells<false>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
-                              pred, w);
+                              cv.run_end, pred, w);
     });
   if (active && !dense) walk_cells<false>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, w);
   if (c.count_only) {
@@ -1036,7 +1025,7 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   if (c.coop)
     for_each_key_group(dense && ok, L.key, [&](int leader, bool mine) {
       group_walk_cells<true>(group_of(c, cv, L, leader), mine, s, p.x, p.y, p.z, c, cv.start,
-                             pred, f);
+                             cv.run_end, pred, f);
     });
   if (ok && !dense) walk_cells<true>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, f);
   if (L.active) {
