@@ -231,13 +231,13 @@ def bench_ensemble(args) -> dict:
     }
 
 
-def adjoint_run(args, steps, coefficients) -> dict:
+def adjoint_run(args, steps, coefficients, every) -> dict:
     """Differentiable rollout: the 65,536-particle block (64x32x32, d=0.05,
     +-0.1d jitter) in [0,4]x[0,3]x[0,2], s_corr + XSPH, no vorticity,
     per-particle v0 ~ N(0, 0.5^2), `steps` substeps.  Loss = mean squared
     distance of the final positions to a target cloud (a forward run from a
     second seeded velocity field).  Times a plain forward rollout, the
-    recorded forward (checkpoint every `args.every`) and the reverse pass
+    recorded forward (checkpoint every `every` substeps) and the reverse pass
     (which recomputes all but the last segment)."""
     sc = scenes.rollout_scene(steps=steps, coefficients=coefficients)
     tg = scenes.rollout_scene(seed=6, steps=steps, coefficients=coefficients)
@@ -261,10 +261,10 @@ def adjoint_run(args, steps, coefficients) -> dict:
     forward()  # warm-up (capacity growth, code load)
     t_fwd = min(forward() for _ in range(2))
     s.upload(sc.state.pos, sc.state.vel)
-    s.forward_record(steps, args.every)  # warm-up of the recording buffers
+    s.forward_record(steps, every)  # warm-up of the recording buffers
     s.backward(np.zeros((3, n), np.float32))
     s.upload(sc.state.pos, sc.state.vel)
-    t_rec, _ = timed(lambda: s.forward_record(steps, args.every))
+    t_rec, _ = timed(lambda: s.forward_record(steps, every))
     fin = s.download()
     diff = fin.pos.astype(np.float64) - target
     loss = float(np.mean(np.sum(diff * diff, axis=0)))
@@ -276,6 +276,7 @@ def adjoint_run(args, steps, coefficients) -> dict:
     return {
         "coefficients": coefficients,
         "particles": n,
+        "checkpoint_every": every,
         "ratio": (t_rec + t_bwd) / t_fwd,
         "forward_s": t_fwd,
         "recorded_forward_s": t_rec,
@@ -298,8 +299,11 @@ def bench_adjoint(args) -> dict:
     the dam-break coefficients, whose 100-substep gradient overflows float32
     (reported, not the headline)."""
     steps = args.steps
-    main_run = adjoint_run(args, steps, "reference")
-    stated = adjoint_run(args, steps, "stated")
+    # the whole rollout recorded as one segment (nothing recomputed; 2.4 GB at
+    # 65,536 particles), and the same with a checkpoint every `args.every`
+    main_run = adjoint_run(args, steps, "reference", steps)
+    ckpt_run = adjoint_run(args, steps, "reference", args.every)
+    stated = adjoint_run(args, steps, "stated", steps)
     return {
         "metric": "adjoint/forward time ratio (gradient of a 100-step rollout vs the rollout)",
         "value": main_run["ratio"],
@@ -315,8 +319,9 @@ def bench_adjoint(args) -> dict:
         "data": "synthetic (seeded jittered lattice, seeded N(0, 0.5^2) velocities)",
         "config": {"workload": "rollout 64x32x32 s_corr+XSPH, 4 iters, box 4x3x2, "
                                "eps 600, k 5e-5, c 2e-4",
-                   "particles": main_run["particles"], "checkpoint_every": args.every},
+                   "particles": main_run["particles"], "checkpoint_every": steps},
         "run": main_run,
+        "checkpointed_run": ckpt_run,
         "stated_coefficients_run": stated,
     }
 
@@ -339,7 +344,8 @@ def main():
                     default="million_block",
                     help="ensemble: batched + hipGraph against a per-scene loop; adjoint: "
                          "gradient of a rollout against the rollout (each its own JSON line)")
-    ap.add_argument("--every", type=int, default=10, help="adjoint checkpoint interval")
+    ap.add_argument("--every", type=int, default=10,
+                    help="checkpoint interval of the adjoint's checkpointed run")
     ap.add_argument("--autotune", type=int, default=2, metavar="TRIAL_STEPS",
                     help="autotune the launch configuration on the warmed-up state before "
                          "the timed region (0: off)")
