@@ -103,13 +103,14 @@ def test_all_particles_at_one_point():
 
 
 @pytest.mark.parametrize("flags", [False, True])
-def test_repeated_waves_are_skipped_exactly(flags):
-    """The default gather mode skips 64-slot waves that repeat the wave before
-    them (same position, velocity and cell, bit for bit) and copies the first
-    wave's results.  Long runs of identical particles in one cell, broken by a
-    few others at nearby points (so that lists differ between runs), must
-    still give the reference's results bit for bit, with and without the
-    skip."""
+def test_runs_of_identical_particles_exactly(flags):
+    """The default gather mode computes each run of identical consecutive
+    slots (same position, velocity and cell, bit for bit) once and copies the
+    run's results to its other slots; walks take long runs as one step.  Long
+    runs of identical particles in one cell, broken by a few others at nearby
+    points (so that lists differ between runs), must still give the
+    reference's results bit for bit, with and without runs (diagnostics turn
+    them off)."""
     p = scenes.base_params((1.0, 1.0, 1.0), flags)
     rng = np.random.default_rng(11)
     big = np.repeat(np.array([[0.512], [0.507], [0.503]], np.float32), 1200, axis=1)
@@ -123,7 +124,7 @@ def test_repeated_waves_are_skipped_exactly(flags):
     vel = np.zeros_like(pos)
     vel[1] = -0.5
     o = oracle.step(p, pos, vel, nsteps=3)
-    for diag in (False, True):  # diagnostics turn the skip off
+    for diag in (False, True):  # diagnostics turn runs off
         with hip.HipSolver(p, [pos.shape[1]], record_diag=diag) as s:
             s.upload(pos, vel)
             s.step(3)
@@ -135,9 +136,9 @@ def test_repeated_waves_are_skipped_exactly(flags):
 @pytest.mark.parametrize("flags", [False, True])
 def test_coincident_clusters_bitwise_with_skips(flags, mode):
     """Without diagnostics, particles of dense cells store no neighbor list
-    (their passes retest the distance predicate) and repeated waves are
-    skipped; results stay bitwise and the pair statistics still count every
-    particle's neighbors."""
+    (their passes retest the distance predicate) and runs of identical slots
+    are computed once; results stay bitwise and the pair statistics still
+    count every particle's neighbors."""
     p = scenes.base_params((1.0, 1.0, 1.0), flags)
     pos = _clusters()
     vel = np.zeros_like(pos)
