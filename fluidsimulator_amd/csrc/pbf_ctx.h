@@ -41,7 +41,7 @@ struct Ctx {
   // launch knobs
   int block = 256;
   int xcd_remap = 1;
-  int scan_blocks = 512;
+  int scan_blocks = 1024;  // workgroups of the device scans (the partials scan takes up to 1024)
   bool record_diag = false;
   int wave_mode = 2;
   int brick_shift = 2;  // log2 of the cells per brick edge of the cell table

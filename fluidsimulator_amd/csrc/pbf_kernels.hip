@@ -264,37 +264,35 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
   // the retry; only the bounding boxes are reset.
   const bool was_fatal = (*flags & kFlagFatal) != 0u;
   if (threadIdx.x == 0) carry = 0;
-  __shared__ int red[6][256 / kWave];
   // Fold each scene's tile partials into its box (which already holds the
-  // atomics of tiles that straddle two scenes).
-  for (int sc = 0; sc < n_scenes; ++sc) {
-    const int a = seg[sc], b = seg[sc + 1];
-    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-    if (b > a) {
-      for (int t = a / tile_size + static_cast<int>(threadIdx.x); t <= (b - 1) / tile_size;
-           t += blockDim.x) {
-        const int* p = tile_bbox + 6 * t;
-        v[0] = min(v[0], p[0]), v[1] = min(v[1], p[1]), v[2] = min(v[2], p[2]);
-        v[3] = max(v[3], p[3]), v[4] = max(v[4], p[4]), v[5] = max(v[5], p[5]);
+  // atomics of tiles that straddle two scenes): `per` wavefronts per scene,
+  // each folding every per-th stretch of 64 tiles.
+  {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+    const int w = static_cast<int>(threadIdx.x) / kWave;
+    const int per = n_scenes >= nw ? 1 : nw / n_scenes;
+    for (int sc = w / per; sc < n_scenes; sc += nw / per) {
+      const int a = seg[sc], b = seg[sc + 1];
+      int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+      if (b > a) {
+        for (int t = a / tile_size + (w % per) * kWave + lane; t <= (b - 1) / tile_size;
+             t += per * kWave) {
+          const int* p = tile_bbox + 6 * t;
+          v[0] = min(v[0], p[0]), v[1] = min(v[1], p[1]), v[2] = min(v[2], p[2]);
+          v[3] = max(v[3], p[3]), v[4] = max(v[4], p[4]), v[5] = max(v[5], p[5]);
+        }
+      }
+      for (int k = 0; k < 3; ++k) v[k] = wave_min(v[k]);
+      for (int k = 3; k < 6; ++k) v[k] = wave_max(v[k]);
+      if (lane == 0) {
+        int* bb = bbox + 6 * sc;
+        for (int k = 0; k < 3; ++k) atomicMin(bb + k, v[k]);
+        for (int k = 3; k < 6; ++k) atomicMax(bb + k, v[k]);
       }
     }
-    for (int k = 0; k < 3; ++k) v[k] = wave_min(v[k]);
-    for (int k = 3; k < 6; ++k) v[k] = wave_max(v[k]);
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    if (lane == 0)
-      for (int k = 0; k < 6; ++k) red[k][w] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int q = 1; q < static_cast<int>(blockDim.x) / kWave; ++q) {
-        for (int k = 0; k < 3; ++k) v[k] = min(v[k], red[k][q]);
-        for (int k = 3; k < 6; ++k) v[k] = max(v[k], red[k][q]);
-      }
-      int* bb = bbox + 6 * sc;
-      for (int k = 0; k < 3; ++k) bb[k] = min(bb[k], v[k]);
-      for (int k = 3; k < 6; ++k) bb[k] = max(bb[k], v[k]);
-    }
-    __syncthreads();
   }
+  __syncthreads();
   for (int b0 = 0; b0 < n_scenes; b0 += blockDim.x) {
     const int s = b0 + threadIdx.x;
     long long nc = 0;
@@ -444,7 +442,8 @@ __global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partia
 }
 
 // out[i] = exclusive prefix; out[len] = grand total.  zero_in clears `in`
-// (the cell histogram is reused by the next substep).
+// (the cell histogram is reused by the next substep).  Each thread takes
+// four consecutive elements per block scan.
 __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
                                                    const int* __restrict__ d_len,
                                                    const int* __restrict__ partial,
@@ -458,14 +457,25 @@ __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
   const int b0 = blockIdx.x * chunk;
   const int b1 = min(len, b0 + chunk);
   int run = partial[blockIdx.x];
-  for (int t0 = b0; t0 < b1; t0 += blockDim.x) {
-    const int i = t0 + threadIdx.x;
-    const int v = (i < b1) ? in[i] : 0;
+  constexpr int kPer = 4;
+  for (int t0 = b0; t0 < b1; t0 += kPer * static_cast<int>(blockDim.x)) {
+    const int i0 = t0 + kPer * static_cast<int>(threadIdx.x);
+    int v[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      v[q] = (i0 + q < b1) ? in[i0 + q] : 0;
+      sum += v[q];
+    }
     int total;
-    const int e = block_excl_scan(v, lds, &total);
-    if (i < b1) {
-      out[i] = run + e;
-      if (zero_in) in[i] = 0;
+    int e = run + block_excl_scan(sum, lds, &total);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (i0 + q < b1) {
+        out[i0 + q] = e;
+        if (zero_in) in[i0 + q] = 0;
+      }
+      e += v[q];
     }
     run += total;
   }
