@@ -676,10 +676,19 @@ __device__ __forceinline__ void walk_cells(int s, float px, float py, float pz, 
         const int k = cell_key(rx + dx, ry + dy, rz + dz, g);
         const int a = start[k];
         const int b = start[k + 1];
-        for (int t = a; t < b; ++t) {
-          if (t == s) continue;
-          if (near0(make_float3(px, py, pz), ld3(pred, t), c)) w.add<kFill>(t);
+        const float3 me = make_float3(px, py, pz);
+        int t = a;
+        // four loads in flight before any test; tests and appends stay in order
+        for (; t + 4 <= b; t += 4) {
+          const float3 q0 = ld3(pred, t), q1 = ld3(pred, t + 1);
+          const float3 q2 = ld3(pred, t + 2), q3 = ld3(pred, t + 3);
+          if (t != s && near0(me, q0, c)) w.add<kFill>(t);
+          if (t + 1 != s && near0(me, q1, c)) w.add<kFill>(t + 1);
+          if (t + 2 != s && near0(me, q2, c)) w.add<kFill>(t + 2);
+          if (t + 3 != s && near0(me, q3, c)) w.add<kFill>(t + 3);
         }
+        for (; t < b; ++t)
+          if (t != s && near0(me, ld3(pred, t), c)) w.add<kFill>(t);
       }
   w.flush<kFill>();
 }
