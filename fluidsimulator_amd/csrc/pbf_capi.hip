@@ -633,7 +633,10 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.off1, static_cast<size_t>(n) + 1);
   ALLOC(c.off2, static_cast<size_t>(n) + 1);
   // every dense cell (>= 32 slots) pads its runs by at most 63 lanes
+  // (n lanes always hold the packed layout; more than 3n + 64 is never used)
   c.lane_cap = 3ll * n + 64;
+  if (cfg.lane_capacity > 0)
+    c.lane_cap = std::min<long long>(c.lane_cap, std::max<long long>(cfg.lane_capacity, n));
   ALLOC(c.lane_rep, static_cast<size_t>(c.lane_cap));
   ALLOC(c.lane_info, 3);
   ALLOC(c.n_dev, 1);
@@ -1128,6 +1131,7 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
       best->brick_edge = 1 << cur.bshift;
       best->cell_capacity = c.cell_cap;
       best->neighbor_capacity = c.nbr_cap;
+      best->lane_capacity = c.lane_cap;
       best->device = c.device;
       best->record_diag = c.record_diag ? 1 : 0;
     }

@@ -43,7 +43,7 @@ class FluidHipConfig(C.Structure):
     _fields_ = [
         ("cell_capacity", C.c_int64), ("neighbor_capacity", C.c_int64), ("device", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("record_diag", C.c_int32),
-        ("wave_mode", C.c_int32), ("brick_edge", C.c_int32),
+        ("wave_mode", C.c_int32), ("brick_edge", C.c_int32), ("lane_capacity", C.c_int64),
     ]
 
 
@@ -166,7 +166,8 @@ class HipSolver:
     counts: particle count per scene (defaults to [state.n] via upload)."""
 
     def __init__(self, params, counts, *, cell_capacity=0, neighbor_capacity=0, block_size=256,
-                 xcd_remap=True, record_diag=False, device=0, wave_mode=0, brick_edge=0):
+                 xcd_remap=True, record_diag=False, device=0, wave_mode=0, brick_edge=0,
+                 lane_capacity=0):
         self.lib = load_library()
         plist = list(params) if isinstance(params, (list, tuple)) else [params]
         counts = [int(c) for c in counts]
@@ -182,7 +183,7 @@ class HipSolver:
         cnt = (C.c_int64 * self.n_scenes)(*counts)
         cfg = FluidHipConfig(int(cell_capacity), int(neighbor_capacity), int(device),
                              int(block_size), int(bool(xcd_remap)), int(bool(record_diag)),
-                             int(wave_mode), int(brick_edge))
+                             int(wave_mode), int(brick_edge), int(lane_capacity))
         h = C.c_void_p()
         _check(self.lib.fluid_hip_create(C.byref(h), self.n_scenes, cnt, arr, C.byref(cfg)))
         self.h = h

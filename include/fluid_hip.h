@@ -86,6 +86,11 @@ typedef struct fluid_hip_config {
                                  3 shared tiles without the coincident-tile shortcut */
   int32_t brick_edge;         /* cells per brick edge of the cell table's Z-order:
                                  0 default (4), 1, 2, 4 or 8 */
+  int64_t lane_capacity;      /* lanes of the gather passes (runs of identical
+                                 particles, each dense cell from a wave boundary):
+                                 0 default (3n + 64), otherwise clamped to
+                                 [n, 3n + 64]; a layout that does not fit
+                                 packs one lane per run */
 } fluid_hip_config;
 
 typedef struct fluid_hip_ctx fluid_hip_ctx;

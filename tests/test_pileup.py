@@ -151,3 +151,30 @@ def test_coincident_clusters_bitwise_with_skips(flags, mode):
         pairs = s.stats()["pairs"]
     assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"])
     assert pairs == int(o["nbr_count"].astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("flags", [False, True])
+def test_lane_map_fallback_bitwise(flags):
+    """Dense cells of one run each need a whole wave of lanes apiece; with the
+    lane capacity at n that layout does not fit, and the passes fall back to
+    one packed lane per run.  Both layouts give the reference's results bit
+    for bit."""
+    p = scenes.base_params((1.0, 1.0, 1.0), flags)
+    rng = np.random.default_rng(21)
+    pts = []
+    for k in range(24):  # 24 cells of 40 identical particles each
+        c = (0.15 + 0.11 * (k % 6), 0.25 + 0.13 * (k // 6), 0.5)
+        pts.append(np.repeat(np.array(c, np.float32)[:, None], 40, axis=1))
+    pts.append((0.5 + 0.15 * rng.standard_normal((3, 200))).astype(np.float32).clip(0.05, 0.95))
+    pos = np.concatenate(pts, axis=1)
+    pos = np.ascontiguousarray(pos[:, rng.permutation(pos.shape[1])])
+    vel = np.zeros_like(pos)
+    vel[1] = -0.5
+    o = oracle.step(p, pos, vel, nsteps=3)
+    n = pos.shape[1]
+    for cap in (0, n):
+        with hip.HipSolver(p, [n], lane_capacity=cap) as s:
+            s.upload(pos, vel)
+            s.step(3)
+            st = s.download()
+        assert same_bits(st.pos, o["pos"]) and same_bits(st.vel, o["vel"]), cap
