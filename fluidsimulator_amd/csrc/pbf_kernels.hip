@@ -1185,7 +1185,7 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
           int npts = 0;
           if (c.coop_bulk && tile_points(pos_of(tile.p), m, cps, cms, npts)) {
             // Candidates at a few points: walk maximal runs of consecutive
-            // candidates at one point, each like a one-point tile.
+            // candidates at one point; each run hoists the pair geometry once.
             const bool tf = tflag(tile, m);
             unsigned long long rest = any;
             while (rest != 0ull) {
@@ -1431,7 +1431,7 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
     }
   };
   float3 gp = p;      // the group's position
-  LTerm tile_t = run;  // the term of the current one-point tile
+  LTerm tile_t = run;  // the term of the current run at one point
   gather(
       c, nv, cv, L, [&](int t) { return EPos{ld3(P, t)}; },
       pair_ops([&](int leader) { return same_bits(bcast(p, leader), p); },
