@@ -99,10 +99,13 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
   return neg ? -x : x;
 }
 
-// add_repeat for hot loops: runs of up to 32 terms are added one by one in
-// place (a measured cut-over), longer ones in O(binades) by the call.
+// Runs of up to this many terms are added one by one in place (a measured
+// cut-over); longer ones in O(binades) by add_repeat.
+constexpr int kAddInPlace = 32;
+
+// add_repeat for hot loops, with the in-place cut-over.
 __host__ __device__ inline float add_k(float x, float w, int k) {
-  if (k <= 32) {
+  if (k <= kAddInPlace) {
     for (int i = 0; i < k; ++i) x += w;
     return x;
   }

@@ -1209,6 +1209,15 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
                 if (mode == 2) on_count(__popcll(mr));
                 continue;
               }
+              // a run whose candidates hold one record gives one term, k times
+              const auto q0 = bcast(tile, j);
+              if (__all(((run >> lane) & 1ull) == 0ull || same_rec(tile, q0))) {
+                if (mode == 0)
+                  pair_rep(q0, __popcll(mr));
+                else if (mode == 2)
+                  on_count(__popcll(mr));
+                continue;
+              }
               for (unsigned long long b = run; b != 0ull; b &= b - 1ull) {
                 const int jj = __builtin_ctzll(b);
                 const auto q = bcast(tile, jj);
@@ -1350,12 +1359,21 @@ __device__ __forceinline__ LTerm bcast(const LTerm& t, int j) {
 __device__ __forceinline__ V3Term bcast(const V3Term& t, int j) {
   return V3Term{bcast(t.x, j), bcast(t.y, j), bcast(t.z, j), bcast(t.on, j)};
 }
-// k successive additions of one 3-vector term, exactly (exact_sum.h).
+// k successive additions of one 3-vector term, exactly (exact_sum.h): short
+// runs in one loop over the components, long ones by the repeated addition.
 __device__ __forceinline__ void add_rep3(float& x, float& y, float& z, const V3Term& t, int k) {
   if (!t.on) return;
-  x = add_k(x, t.x, k);
-  y = add_k(y, t.y, k);
-  z = add_k(z, t.z, k);
+  if (k <= kAddInPlace) {
+    for (int i = 0; i < k; ++i) {
+      x += t.x;
+      y += t.y;
+      z += t.z;
+    }
+    return;
+  }
+  x = add_repeat(x, t.x, k);
+  y = add_repeat(y, t.y, k);
+  z = add_repeat(z, t.z, k);
 }
 
 __device__ __forceinline__ bool same_term(const LTerm& a, const LTerm& b) {
@@ -1411,12 +1429,28 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
   int run_n = 0;
   auto flush = [&]() {
     if (run_n == 0) return;
-    rho = add_k(rho, run.w, run_n);
-    if (run.grad) {
-      gsx = add_k(gsx, run.gx, run_n);
-      gsy = add_k(gsy, run.gy, run_n);
-      gsz = add_k(gsz, run.gz, run_n);
-      sg2 = add_k(sg2, run.j2, run_n);
+    if (run_n <= kAddInPlace) {
+      // one loop for every accumulator (each still receives its own
+      // additions in order)
+      if (run.grad) {
+        for (int i = 0; i < run_n; ++i) {
+          rho += run.w;
+          gsx += run.gx;
+          gsy += run.gy;
+          gsz += run.gz;
+          sg2 += run.j2;
+        }
+      } else {
+        for (int i = 0; i < run_n; ++i) rho += run.w;
+      }
+    } else {
+      rho = add_repeat(rho, run.w, run_n);
+      if (run.grad) {
+        gsx = add_repeat(gsx, run.gx, run_n);
+        gsy = add_repeat(gsy, run.gy, run_n);
+        gsz = add_repeat(gsz, run.gz, run_n);
+        sg2 = add_repeat(sg2, run.j2, run_n);
+      }
     }
     run_n = 0;
   };
