@@ -38,6 +38,8 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
     w = -w;
   }
   auto plain = [neg, w](float v) { return neg ? -((-v) + (-w)) : v + w; };
+  // parity of an integer-valued double (exact below 2^53)
+  auto odd = [](double v) { return v - 2.0 * std::floor(0.5 * v) != 0.0; };
   const double kTop = 16777216.0;    // 2^24: one past the largest multiple of u in the binade
   const double kBottom = 8388608.0;  // 2^23: the binade's lowest multiple of u
   while (k > 0) {
@@ -52,9 +54,10 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
     uint32_t ab;
     std::memcpy(&ab, &ax, sizeof(ab));
     const int e = static_cast<int>((ab >> 23) & 0xffu) - 126;
-    const double u = std::ldexp(1.0, e - 24);
-    const double M = static_cast<double>(ax) / u;  // integer in [2^23, 2^24)
-    const double W = static_cast<double>(w) / u;
+    // in units of u = 2^(e-24), the binade's spacing (power-of-two scalings
+    // are exact)
+    const double M = std::ldexp(static_cast<double>(ax), 24 - e);  // integer in [2^23, 2^24)
+    const double W = std::ldexp(static_cast<double>(w), 24 - e);
     const bool up = x > 0.0f;  // |x| grows; otherwise it shrinks toward zero
     if (up ? !(M + W < kTop) : !(M - W >= kBottom)) {
       x = plain(x);  // this addition leaves the binade
@@ -69,9 +72,9 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
       // halfway: the nearest even multiple, then q + (q odd) per addition
       const double a = up ? M + q : M - q;
       const double b = up ? a + 1.0 : a - 1.0;
-      Mc = (std::fmod(a, 2.0) == 0.0) ? a : b;
+      Mc = (odd(a) ? b : a);
       --k;
-      d = q + std::fmod(q, 2.0);
+      d = q + (odd(q) ? 1.0 : 0.0);
     } else {
       d = (f < 0.5) ? q : q + 1.0;
     }
@@ -93,7 +96,7 @@ __host__ __device__ inline __attribute__((noinline)) float add_repeat(float x, f
         k -= static_cast<int>(t);
       }
     }
-    const float mag = static_cast<float>(Mc * u);
+    const float mag = static_cast<float>(std::ldexp(Mc, e - 24));
     x = up ? mag : -mag;
   }
   return neg ? -x : x;
