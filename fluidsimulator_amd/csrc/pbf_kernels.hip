@@ -873,6 +873,25 @@ __device__ __forceinline__ bool tile_points(const float3& q, int m, float3 (&cp)
   return __ballot(pend) == 0ull;
 }
 
+// Like tile_points, but keeps no per-point arrays: cls is the index of the
+// point this lane's candidate sits at (-1 for lanes past m).  A point's
+// position is its candidates' own, and its mask a ballot of cls.
+template <int K>
+__device__ __forceinline__ bool tile_point_ids(const float3& q, int m, int& cls) {
+  const int lane = threadIdx.x & (kWave - 1);
+  bool pend = lane < m;
+  cls = -1;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const unsigned long long pm = __ballot(pend);
+    if (pm == 0ull) break;
+    const bool eq = pend && same_bits(q, bcast(q, __builtin_ctzll(pm)));
+    if (eq) cls = k;
+    pend = pend && !eq;
+  }
+  return __ballot(pend) == 0ull;
+}
+
 // This lane's neighbors among the m candidates of a tile (bit j: candidate
 // base + j, whose p^0 this lane holds in t0), by the list predicate near0;
 // `any` receives the candidates that some walking lane takes (a superset).
@@ -1180,24 +1199,17 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
           unsigned long long any;
           const unsigned long long mine =
               tile_members(lc, me0, in, tt.p0, m, L.s - base, on, c, any);
-          float3 cps[kTilePoints];
-          unsigned long long cms[kTilePoints];
-          int npts = 0;
-          if (c.coop_bulk && tile_points(pos_of(tile.p), m, cps, cms, npts)) {
+          int cls;
+          if (c.coop_bulk && tile_point_ids<kTilePoints>(pos_of(tile.p), m, cls)) {
             // Candidates at a few points: walk maximal runs of consecutive
             // candidates at one point; each run hoists the pair geometry once.
             const bool tf = tflag(tile, m);
             unsigned long long rest = any;
             while (rest != 0ull) {
               const int j = __builtin_ctzll(rest);
-              float3 cpk = cps[0];
-              unsigned long long cmk = cms[0];
-#pragma unroll
-              for (int k = 1; k < kTilePoints; ++k)
-                if (k < npts && ((cms[k] >> j) & 1ull)) {
-                  cpk = cps[k];
-                  cmk = cms[k];
-                }
+              // the run's point: candidate j's position; its candidates: one ballot
+              const float3 cpk = bcast(pos_of(tile.p), j);
+              const unsigned long long cmk = __ballot(cls == __builtin_amdgcn_readlane(cls, j));
               const unsigned long long other = rest & ~cmk;
               const unsigned long long upto =
                   other != 0ull ? ((1ull << __builtin_ctzll(other)) - 1ull) : ~0ull;
