@@ -823,12 +823,13 @@ __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
 __device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
 
 // Lanes of a walk grouped by their p^0 (bit for bit).  Lanes that share p^0
-// share their neighbors, but for themselves.
+// share their neighbors, but for themselves.  A class's p^0 is that of its
+// first lane (kept nowhere else: uniform values live across the whole walk
+// would crowd out the scalar registers).
 struct LaneClasses {
-  float3 p[kP0Classes];  // wavefront-uniform
-  int cls;               // this lane's class, -1 when none
-  int n;                 // classes found
-  bool all;              // every walking lane has a class
+  int cls;   // this lane's class, -1 when none
+  int n;     // classes found
+  bool all;  // every walking lane has a class
 };
 
 __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, bool on) {
@@ -840,11 +841,11 @@ __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, 
 #pragma unroll
     for (int k = 0; k < kP0Classes; ++k) {
       const unsigned long long pm = __ballot(pend);
-      lc.p[k] = bcast(me0, pm != 0ull ? __builtin_ctzll(pm) : 0);
-      const bool eq = pend && same_bits(me0, lc.p[k]);
+      if (pm == 0ull) break;
+      const bool eq = pend && same_bits(me0, bcast(me0, __builtin_ctzll(pm)));
       if (eq) lc.cls = k;
       pend = pend && !eq;
-      lc.n += pm != 0ull ? 1 : 0;
+      ++lc.n;
     }
   }
   lc.all = __ballot(pend) == 0ull;
@@ -912,7 +913,8 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
 #pragma unroll
     for (int k = 0; k < kP0Classes; ++k) {
       if (k < lc.n) {
-        const unsigned long long mk = __ballot(lane < m && near0(lc.p[k], t0, c));
+        const float3 pk = bcast(me0, __builtin_ctzll(__ballot(lc.cls == k)));
+        const unsigned long long mk = __ballot(lane < m && near0(pk, t0, c));
         any |= mk;
         if (lc.cls == k) mine = mk;
       }
