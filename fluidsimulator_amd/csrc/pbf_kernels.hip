@@ -852,31 +852,11 @@ __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, 
   return lc;
 }
 
-// The distinct positions (bit for bit) of the m candidates of a tile, up to
-// K of them: cp[k] and the mask of the candidates there, cm[k]; n is the
-// number found.  True when every candidate is at one of them.
-template <int K>
-__device__ __forceinline__ bool tile_points(const float3& q, int m, float3 (&cp)[K],
-                                            unsigned long long (&cm)[K], int& n) {
-  const int lane = threadIdx.x & (kWave - 1);
-  bool pend = lane < m;
-  n = 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const unsigned long long pm = __ballot(pend);
-    if (pm == 0ull) break;
-    cp[k] = bcast(q, __builtin_ctzll(pm));
-    const bool eq = pend && same_bits(q, cp[k]);
-    cm[k] = __ballot(eq);
-    pend = pend && !eq;
-    ++n;
-  }
-  return __ballot(pend) == 0ull;
-}
-
-// Like tile_points, but keeps no per-point arrays: cls is the index of the
-// point this lane's candidate sits at (-1 for lanes past m).  A point's
-// position is its candidates' own, and its mask a ballot of cls.
+// Groups the m candidates of a tile by position (bit for bit), up to K
+// positions: cls is the index of the position this lane's candidate sits at
+// (-1 for lanes past m).  True when every candidate is at one of them.  A
+// position's point is its candidates' own, and its mask a ballot of cls (no
+// per-position arrays, which would crowd the scalar registers).
 template <int K>
 __device__ __forceinline__ bool tile_point_ids(const float3& q, int m, int& cls) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -922,18 +902,16 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
     done = true;
   }
   if (on && !done) {
-    // distinct candidate positions of the tile
-    float3 cp[kP0Classes];
-    unsigned long long cm[kP0Classes];
-    int n = 0;
-    if (tile_points(t0, m, cp, cm, n)) {
+    // distinct candidate positions of the tile: one test per position and lane
+    int ccls;
+    if (tile_point_ids<kP0Classes>(t0, m, ccls)) {
 #pragma unroll
       for (int k = 0; k < kP0Classes; ++k) {
-        if (k < n) {
-          const bool nk = in && near0(me0, cp[k], c);
-          if (nk) mine |= cm[k];
-          if (__ballot(nk) != 0ull) any |= cm[k];
-        }
+        const unsigned long long ck = __ballot(ccls == k);
+        if (ck == 0ull) break;
+        const bool nk = in && near0(me0, bcast(t0, __builtin_ctzll(ck)), c);
+        if (nk) mine |= ck;
+        if (__ballot(nk) != 0ull) any |= ck;
       }
       done = true;
     }
