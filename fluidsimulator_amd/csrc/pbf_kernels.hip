@@ -852,6 +852,15 @@ __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, 
   return lc;
 }
 
+// The maximal run of consecutive candidates of `rest` (a set of a tile's
+// candidates, in order) that starts at its first candidate and holds that
+// candidate's record: eq has bit j set where candidate j holds it.
+__device__ __forceinline__ unsigned long long record_run(unsigned long long rest,
+                                                         unsigned long long eq) {
+  const unsigned long long other = rest & ~eq;
+  return other != 0ull ? rest & ((1ull << __builtin_ctzll(other)) - 1ull) : rest;
+}
+
 // Groups the m candidates of a tile by position (bit for bit), up to K
 // positions: cls is the index of the position this lane's candidate sits at
 // (-1 for lanes past m).  True when every candidate is at one of them.  A
@@ -1119,6 +1128,9 @@ struct TileP0 {
 //    one point) get the same term from each candidate: in their walk, lane
 //    j computes candidate j's term once and every lane adds the broadcast
 //    terms of its own neighbors in order;
+//  * other tiles are walked as maximal runs of consecutive candidates
+//    holding one record (bit for bit), each with prep at its point and
+//    then, in mode 0, pair_rep once for the lane's members in the run;
 //  * other lanes walk their own run-length-encoded lists.
 // Every lane belongs to exactly one walk, so each sum still runs over the
 // lane's list in order.
@@ -1232,10 +1244,27 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
             }
             return;
           }
-          for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
-            const int j = __builtin_ctzll(b);
-            const auto q = bcast(tile, j);
-            if ((mine >> j) & 1ull) ops.pair(q);
+          if (!c.coop_bulk) {
+            for (unsigned long long b = any; b != 0ull; b &= b - 1ull) {
+              const int j = __builtin_ctzll(b);
+              const auto q = bcast(tile, j);
+              if ((mine >> j) & 1ull) ops.pair(q);
+            }
+            return;
+          }
+          // other tiles: maximal runs of consecutive candidates holding one
+          // record (one position), each through prep once per lane
+          const bool tf = tflag(tile, m);
+          for (unsigned long long rr = any; rr != 0ull;) {
+            const auto q0 = bcast(tile, __builtin_ctzll(rr));
+            const unsigned long long sub = record_run(rr, __ballot(same_rec(tile, q0)));
+            rr &= ~sub;
+            const unsigned long long mr = mine & sub;
+            const int mode = mr != 0ull ? prep(pos_of(q0.p), tf) : 1;
+            if (mode == 0)
+              pair_rep(q0, __popcll(mr));
+            else if (mode == 2)
+              on_count(__popcll(mr));
           }
         });
     flush();
