@@ -826,27 +826,32 @@ __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
 __device__ __forceinline__ float3 pos_of(const float4& q) { return make_float3(q.x, q.y, q.z); }
 
 // Lanes of a walk grouped by their p^0 (bit for bit).  Lanes that share p^0
-// share their neighbors, but for themselves.  A class's p^0 is that of its
-// first lane (kept nowhere else: uniform values live across the whole walk
-// would crowd out the scalar registers).
+// share their neighbors, but for themselves.  Class k's p^0 is kept in lane
+// k of `pt` (uniform values live across the whole walk would crowd out the
+// scalar registers), so a tile reads it from a fixed lane.
 struct LaneClasses {
-  int cls;   // this lane's class, -1 when none
-  int n;     // classes found
-  bool all;  // every walking lane has a class
+  int cls;    // this lane's class, -1 when none
+  int n;      // classes found
+  bool all;   // every walking lane has a class
+  float3 pt;  // lane k < n: class k's p^0
 };
 
 __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, bool on) {
+  const int lane = threadIdx.x & (kWave - 1);
   LaneClasses lc;
   lc.cls = -1;
   lc.n = 0;
+  lc.pt = me0;
   bool pend = in;
   if (on) {
 #pragma unroll
     for (int k = 0; k < kP0Classes; ++k) {
       const unsigned long long pm = __ballot(pend);
       if (pm == 0ull) break;
-      const bool eq = pend && same_bits(me0, bcast(me0, __builtin_ctzll(pm)));
+      const float3 pk = bcast(me0, __builtin_ctzll(pm));
+      const bool eq = pend && same_bits(me0, pk);
       if (eq) lc.cls = k;
+      if (lane == k) lc.pt = pk;
       pend = pend && !eq;
       ++lc.n;
     }
@@ -905,7 +910,7 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
 #pragma unroll
     for (int k = 0; k < kP0Classes; ++k) {
       if (k < lc.n) {
-        const float3 pk = bcast(me0, __builtin_ctzll(__ballot(lc.cls == k)));
+        const float3 pk = bcast(lc.pt, k);
         const unsigned long long mk = __ballot(lane < m && near0(pk, t0, c));
         any |= mk;
         if (lc.cls == k) mine = mk;
@@ -914,17 +919,25 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
     done = true;
   }
   if (on && !done) {
-    // distinct candidate positions of the tile: one test per position and lane
-    int ccls;
-    if (tile_point_ids<kP0Classes>(t0, m, ccls)) {
+    // distinct candidate positions of the tile, each tested once per lane as
+    // it is found
+    bool pend = lane < m;
+    unsigned long long mm = 0ull, an = 0ull;
 #pragma unroll
-      for (int k = 0; k < kP0Classes; ++k) {
-        const unsigned long long ck = __ballot(ccls == k);
-        if (ck == 0ull) break;
-        const bool nk = in && near0(me0, bcast(t0, __builtin_ctzll(ck)), c);
-        if (nk) mine |= ck;
-        if (__ballot(nk) != 0ull) any |= ck;
-      }
+    for (int k = 0; k < kP0Classes; ++k) {
+      const unsigned long long pm = __ballot(pend);
+      if (pm == 0ull) break;
+      const float3 pk = bcast(t0, __builtin_ctzll(pm));
+      const bool eq = pend && same_bits(t0, pk);
+      pend = pend && !eq;
+      const unsigned long long ck = __ballot(eq);
+      const bool nk = in && near0(me0, pk, c);
+      if (nk) mm |= ck;
+      if (__ballot(nk) != 0ull) an |= ck;
+    }
+    if (__ballot(pend) == 0ull) {
+      mine = mm;
+      any = an;
       done = true;
     }
   }
