@@ -860,13 +860,22 @@ __device__ __forceinline__ LaneClasses lane_classes(const float3& me0, bool in, 
   return lc;
 }
 
-// The maximal run of consecutive candidates of `rest` (a set of a tile's
-// candidates, in order) that starts at its first candidate and holds that
-// candidate's record: eq has bit j set where candidate j holds it.
-__device__ __forceinline__ unsigned long long record_run(unsigned long long rest,
-                                                         unsigned long long eq) {
-  const unsigned long long other = rest & ~eq;
-  return other != 0ull ? rest & ((1ull << __builtin_ctzll(other)) - 1ull) : rest;
+// The candidates of `rest` (a set of a tile's candidates) from its first
+// one, j, up to the next run start in `starts` after j.
+__device__ __forceinline__ unsigned long long run_from(unsigned long long rest,
+                                                       unsigned long long starts, int j) {
+  const unsigned long long later = starts & ~((2ull << j) - 1ull);
+  return later != 0ull ? rest & ((1ull << __builtin_ctzll(later)) - 1ull) : rest;
+}
+
+// The value of v in the previous lane (lane 0: lane 63's), by a permute
+// over the whole wavefront; every lane must be active.
+__device__ __forceinline__ float prev_lane(float v) {
+  const int src = ((threadIdx.x & (kWave - 1)) + kWave - 1) & (kWave - 1);
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ float3 prev_lane(const float3& v) {
+  return make_float3(prev_lane(v.x), prev_lane(v.y), prev_lane(v.z));
 }
 
 // Groups the m candidates of a tile by position (bit for bit), up to K
@@ -1269,11 +1278,17 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
             return;
           }
           // other tiles: maximal runs of consecutive candidates holding one
-          // record (one position), each through prep once per lane
+          // record (one position), each through prep once per lane; a run
+          // starts where a candidate's record differs from the previous one's
           const bool tf = tflag(tile, m);
+          // (the permute runs in every lane: inactive lanes give no data)
+          const auto prev = prev_lane(tile);
+          const bool differs = !same_rec(tile, prev);
+          const unsigned long long starts = __ballot(lane < m && (lane == 0 || differs));
           for (unsigned long long rr = any; rr != 0ull;) {
-            const auto q0 = bcast(tile, __builtin_ctzll(rr));
-            const unsigned long long sub = record_run(rr, __ballot(same_rec(tile, q0)));
+            const int j = __builtin_ctzll(rr);
+            const auto q0 = bcast(tile, j);
+            const unsigned long long sub = run_from(rr, starts, j);
             rr &= ~sub;
             const unsigned long long mr = mine & sub;
             const int mode = mr != 0ull ? prep(pos_of(q0.p), tf) : 1;
@@ -1361,6 +1376,21 @@ __device__ __forceinline__ EPosV3 bcast(const EPosV3& e, int j) {
 }
 __device__ __forceinline__ EPosW bcast(const EPosW& e, int j) {
   return EPosW{bcast(e.p, j), bcast(e.w, j)};
+}
+
+// The previous lane's candidate record.
+__device__ __forceinline__ EPos prev_lane(const EPos& e) { return EPos{prev_lane(e.p)}; }
+__device__ __forceinline__ EPosL prev_lane(const EPosL& e) {
+  return EPosL{prev_lane(e.p), prev_lane(e.l)};
+}
+__device__ __forceinline__ EPosV prev_lane(const EPosV& e) {
+  return EPosV{prev_lane(e.p), prev_lane(e.v), prev_lane(e.irho)};
+}
+__device__ __forceinline__ EPosV3 prev_lane(const EPosV3& e) {
+  return EPosV3{prev_lane(e.p), prev_lane(e.v)};
+}
+__device__ __forceinline__ EPosW prev_lane(const EPosW& e) {
+  return EPosW{prev_lane(e.p), prev_lane(e.w)};
 }
 
 // Bitwise equality of candidate records.
