@@ -129,14 +129,20 @@ def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
         short = (m.group(1) + (m.group(2) or "") if m else name.split("(")[0])[:28]
         lines.append(f"{short:28s} {n:7d} {avg / 1e3:12.1f} {tot / 1e6:12.1f} "
                      f"{100.0 * tot / total:6.2f}")
-    lines += ["", f"timed window: last {steps} substeps of each kernel's dispatches:"]
+    lines += ["", f"timed window: last {steps} substeps of each kernel's dispatches",
+              "(dispatches that returned at once, under 1% of the kernel's median, are",
+              "left out: after a capacity overflow the substeps queued behind the failed",
+              "one do no work, and the failed one runs again):"]
     for key, per_step in launches_per_step.items():
         d = [r[0] for r in cur.execute(
             "select duration from kernels where name like ? order by start", (f"%k_{key}%",))]
         take = steps * per_step
         if d and take:
-            w = d[-take:]
-            lines.append(f"  {key:18s} launches {len(w):6d}  avg_us {sum(w) / len(w) / 1e3:12.1f}")
+            med = sorted(d)[len(d) // 2]
+            live = [x for x in d if x >= 0.01 * med]
+            w = live[-take:]
+            lines.append(f"  {key:18s} launches {len(w):6d}  avg_us {sum(w) / len(w) / 1e3:12.1f}"
+                         f"  (skipped {len(d) - len(live)})")
     return "\n".join(lines)
 
 

@@ -267,14 +267,19 @@ int alloc_neighbors(Ctx& c, long long cap) {
   return FLUID_HIP_OK;
 }
 
-int collect_profile(Ctx& c) {
+// Adds the finished launches' times to the per-kernel totals.  Launches of
+// substeps from `keep_steps` on (in queue order) are dropped: after a
+// capacity overflow those substeps did no work or are run again.
+int collect_profile(Ctx& c, int keep_steps = INT_MAX) {
   for (auto& q : c.prof.pending) {
-    float ms = 0.0f;
-    hipError_t e = hipEventElapsedTime(&ms, q.a, q.b);
-    if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
-    auto& slot = c.prof.acc[q.name];
-    slot.first += ms;
-    slot.second += 1;
+    if (q.step < keep_steps) {
+      float ms = 0.0f;
+      hipError_t e = hipEventElapsedTime(&ms, q.a, q.b);
+      if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+      auto& slot = c.prof.acc[q.name];
+      slot.first += ms;
+      slot.second += 1;
+    }
     c.prof.pool.push_back(q.a);
     c.prof.pool.push_back(q.b);
   }
@@ -305,6 +310,7 @@ int queue_steps(Ctx& c, int nsteps, bool use_graph) {
     for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph_exec, c.stream), "hipGraphLaunch");
   } else {
     for (int s = 0; s < nsteps; ++s) {
+      c.prof.step = s;
       hipError_t e = fluidhip::enqueue_substep(c);
       if (e != hipSuccess) return hip_fail(e, "substep launch");
     }
@@ -332,7 +338,7 @@ int read_status(Ctx& c, DevStatus& st) {
                          hipMemcpyDeviceToHost, c.stream),
           "read cell need");
   HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
-  return collect_profile(c);
+  return FLUID_HIP_OK;
 }
 
 // log2 of a brick edge (0 selects the default 4), or -1 when unsupported.
@@ -451,9 +457,11 @@ int run_recorded(Ctx& c, SubTape& t) {
   for (int attempt = 0;; ++attempt) {
     if (attempt >= 16) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
     HIP_TRY(fluidhip::enqueue_reset_flags(c), "reset flags");
+    c.prof.step = 0;
     hipError_t e = fluidhip::enqueue_substep(c, &rec);
     if (e != hipSuccess) return hip_fail(e, "recorded substep launch");
     if ((rc = read_status(c, st))) return rc;
+    if ((rc = collect_profile(c, (st.flags & fluidhip::kFlagFatal) ? 0 : INT_MAX))) return rc;
     if (!(st.flags & fluidhip::kFlagFatal)) break;
     if (st.flags & fluidhip::kFlagRange)
       return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
@@ -792,6 +800,7 @@ int fluid_hip_sync(fluid_hip_ctx* ctx) {
   Ctx& c = ctx->c;
   DevStatus st{};
   if ((rc = read_status(c, st))) return rc;
+  if ((rc = collect_profile(c))) return rc;
   if (st.flags & fluidhip::kFlagRange)
     return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
   if (st.flags & (fluidhip::kFlagCells | fluidhip::kFlagNeighbors))
@@ -816,6 +825,7 @@ int fluid_hip_step(fluid_hip_ctx* ctx, int32_t nsteps, int32_t use_graph) {
     if (st.flags & fluidhip::kFlagFatal)
       done = st.steps_started - ((st.flags & fluidhip::kFlagNeighbors) ? 1 : 0);
     if (c.n == 0) done = remaining;
+    if ((rc = collect_profile(c, done))) return rc;
     for (int s = 0; s < done; ++s) c.time += c.params[0].dt;  // core.cpp:614
     if (done > 0) c.stepped = true;
     remaining -= done;

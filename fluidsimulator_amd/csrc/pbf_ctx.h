@@ -17,8 +17,10 @@ struct Prof {
   struct Pending {
     const char* name;
     hipEvent_t a, b;
+    int step;  // substep of the queue the launch belongs to
   };
   bool on = false;
+  int step = 0;  // substep of the queue being enqueued
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   std::unordered_map<std::string, std::pair<double, long long>> acc;

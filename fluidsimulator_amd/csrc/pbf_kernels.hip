@@ -2187,7 +2187,7 @@ struct Launcher {
       (void)hipEventRecord(a, c.stream);
       f();
       (void)hipEventRecord(b, c.stream);
-      c.prof.pending.push_back({name, a, b});
+      c.prof.pending.push_back({name, a, b, c.prof.step});
     } else {
       f();
     }

@@ -199,6 +199,24 @@ def test_capacity_auto_growth_keeps_results():
     assert same_bits(r["pos"], o["pos"]) and same_bits(r["vel"], o["vel"])
 
 
+def test_profile_counts_each_substep_once_across_capacity_growth():
+    """Per-kernel event totals count the launches of completed substeps
+    only: a substep that overflowed a capacity is run again after the
+    growth, and the substeps queued behind it did no work, so neither
+    counts."""
+    sc = scenes.dam_break(seed=11, dims=(16, 16, 16), flags=True)
+    iters = sc.params.solver_iterations
+    with hip.HipSolver(sc.params, [sc.state.n], cell_capacity=64, neighbor_capacity=64) as s:
+        s.upload(sc.state.pos, sc.state.vel)
+        s.profile(True)
+        s.step(5)
+        grown = s.stats()["grow_events"]
+        counts = {k: s.profile_read(k)[1] for k in ("neighbors", "lambda", "delta", "xsph")}
+        s.profile(False)
+    assert grown >= 1
+    assert counts == {"neighbors": 5, "lambda": 5 * iters, "delta": 5 * iters, "xsph": 5}
+
+
 def test_non_finite_position_reports_range_error_and_keeps_state():
     sc = scenes.dam_break(seed=12, dims=(8, 8, 8))
     pos = sc.state.pos.copy()
