@@ -33,6 +33,8 @@ HIPCC_FLAGS = [
     # on the piled-up million-particle dam break).
     "-mllvm", "-amdgpu-sched-strategy=max-ilp",
     "-Wall", "-Wno-unused-function", "-Wno-unused-value",
+    # per-lane conditions are combined with & on purpose (no branch per term)
+    "-Wno-bitwise-instead-of-logical",
 ]
 
 

@@ -109,6 +109,15 @@ __device__ __forceinline__ float4 bcast(const float4& v, int j) {
   return make_float4(bcast(v.x, j), bcast(v.y, j), bcast(v.z, j), bcast(v.w, j));
 }
 
+// Bit differences of float values: zero exactly when they are equal bit for
+// bit (-0 and +0 differ, a NaN equals itself).  Comparing a record by OR-ing
+// its fields' differences and testing once keeps the test free of branches
+// (a chain of && over per-lane values can compile to one divergent branch
+// per term).
+__device__ __forceinline__ unsigned bit_diff(float a, float b) {
+  return __float_as_uint(a) ^ __float_as_uint(b);
+}
+
 // Bitwise equality of float values (distinguishes -0 and +0, NaN equals itself).
 __device__ __forceinline__ bool same_bits(const float3& a, const float3& b) {
   return __float_as_int(a.x) == __float_as_int(b.x) && __float_as_int(a.y) == __float_as_int(b.y) &&
@@ -1446,8 +1455,8 @@ __device__ __forceinline__ void add_rep3(float& x, float& y, float& z, const V3T
 }
 
 __device__ __forceinline__ bool same_term(const LTerm& a, const LTerm& b) {
-  return a.grad == b.grad && same_bits(a.w, b.w) && same_bits(a.gx, b.gx) &&
-         same_bits(a.gy, b.gy) && same_bits(a.gz, b.gz) && same_bits(a.j2, b.j2);
+  return (static_cast<unsigned>(a.grad ^ b.grad) | bit_diff(a.w, b.w) | bit_diff(a.gx, b.gx) |
+          bit_diff(a.gy, b.gy) | bit_diff(a.gz, b.gz) | bit_diff(a.j2, b.j2)) == 0u;
 }
 
 // ---------------------------------------------------------------- solver
@@ -1484,8 +1493,8 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
     const float dz = ps.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     // coincident: poly6(0), and the f * (+-0) terms leave the sums unchanged
-    const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-    const bool grad = !same && r2 < c.h2;
+    const bool same = (dx == 0.0f) & (dy == 0.0f) & (dz == 0.0f);
+    const bool grad = !same & (r2 < c.h2);
     const float f = grad ? pair_spiky(r2, c) : 0.0f;
     const float gx = f * dx, gy = f * dy, gz = f * dz;
     const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
@@ -1552,7 +1561,7 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
       [&](int cnt) {
         // cnt identical neighbors: each accumulator receives the same term
         // cnt times, in order
-        if (run_n > 0 && !same_term(run, tile_t)) flush();
+        if ((run_n > 0) & !same_term(run, tile_t)) flush();
         run = tile_t;
         run_n += cnt;
       });
