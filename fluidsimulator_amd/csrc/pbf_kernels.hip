@@ -796,39 +796,45 @@ constexpr int kRunStep = 32;
 // m = its length, lane 0 of `tile` holding slot base: every slot of the run
 // holds the same record); otherwise a tile of up to 64 candidates, `tile`
 // holding load(candidate) of lane `lane` (lanes past m repeat the last).
-// The next step's load is issued before a step is processed.
+// The next step's load is issued before a step is processed; the 27 cells'
+// slot ranges (and the run end of each one's first slot) are loaded up
+// front, one cell per lane, instead of one dependent load chain per cell.
 template <typename Load, typename Body>
 __device__ __forceinline__ void group_steps(const Group& gr, const int* __restrict__ start,
                                             const int* __restrict__ run_end, Load&& load,
                                             Body&& body) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int k = __builtin_amdgcn_readfirstlane(
-            cell_key(gr.rx + dx, gr.ry + dy, gr.rz + dz, gr.g));
-        const int a = __builtin_amdgcn_readfirstlane(start[k]);
-        const int b = __builtin_amdgcn_readfirstlane(start[k + 1]);
-        if (a >= b) continue;
-        int t = a;
-        int et = run_end ? __builtin_amdgcn_readfirstlane(run_end[t]) : t;
-        auto next = load(t + min(lane, min(kWave, b - t) - 1));
-        while (t < b) {
-          const int er = min(et, b);
-          const bool whole = run_end != nullptr && er - t >= kRunStep;
-          const int m = whole ? er - t : min(kWave, b - t);
-          const auto cur = next;
-          const int tn = t + m;
-          int en = tn;
-          if (tn < b) {
-            if (run_end) en = __builtin_amdgcn_readfirstlane(run_end[tn]);
-            next = load(tn + min(lane, min(kWave, b - tn) - 1));
-          }
-          body(t, m, cur, whole);
-          t = tn;
-          et = en;
-        }
+  int ca = 0, cb = 0, ce = 0;
+  if (lane < 27) {  // lane = 9 (dz + 1) + 3 (dy + 1) + (dx + 1)
+    const int k = cell_key(gr.rx + lane % 3 - 1, gr.ry + (lane / 3) % 3 - 1,
+                           gr.rz + lane / 9 - 1, gr.g);
+    ca = start[k];
+    cb = start[k + 1];
+    if (run_end != nullptr && ca < cb) ce = run_end[ca];
+  }
+  for (int ci = 0; ci < 27; ++ci) {
+    const int a = __builtin_amdgcn_readlane(ca, ci);
+    const int b = __builtin_amdgcn_readlane(cb, ci);
+    if (a >= b) continue;
+    int t = a;
+    int et = run_end ? __builtin_amdgcn_readlane(ce, ci) : t;
+    auto next = load(t + min(lane, min(kWave, b - t) - 1));
+    while (t < b) {
+      const int er = min(et, b);
+      const bool whole = run_end != nullptr && er - t >= kRunStep;
+      const int m = whole ? er - t : min(kWave, b - t);
+      const auto cur = next;
+      const int tn = t + m;
+      int en = tn;
+      if (tn < b) {
+        if (run_end) en = __builtin_amdgcn_readfirstlane(run_end[tn]);
+        next = load(tn + min(lane, min(kWave, b - tn) - 1));
       }
+      body(t, m, cur, whole);
+      t = tn;
+      et = en;
+    }
+  }
 }
 
 __device__ __forceinline__ float3 pos_of(const float3& q) { return q; }
