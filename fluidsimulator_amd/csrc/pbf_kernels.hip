@@ -366,12 +366,16 @@ constexpr int kDenseCell = 32;
 // lists are built, candidates).
 constexpr int kHeavyList = 256;
 
-// Distinct predicted positions p^0, among the lanes of a walk or among the
-// candidates of a tile, up to which a tile's neighbor sets are decided with
-// one test per position; beyond it each lane tests each candidate.
-// Measured on the piled-up million-particle dam break (substeps/s): 3: 35.7,
-// 4: 39.0, 5: 40.5, 6: 40.9, 8: 40.8, 12: 39.6.
+// Distinct predicted positions p^0 among the lanes of a walk up to which a
+// tile's neighbor sets are decided with one ballot per position, and
+// distinct p^0 among a tile's candidates up to which they are decided with
+// one test per position and lane; beyond both each lane tests each
+// candidate.  Measured on the piled-up million-particle dam break
+// (substeps/s): both at 3: 35.7, 4: 39.0, 5: 40.5, 6: 40.9, 8: 40.8, 12: 39.6;
+// lanes 4 / 8 with candidates at 6: 40.8 / 40.3; candidates 4 / 8 / 10 / 12
+// with lanes at 6: 39.0 / 41.3 / 39.8 / 41.2.
 constexpr int kP0Classes = 6;
+constexpr int kCandPoints = 8;
 
 // Distinct positions of a tile's candidates up to which a pass walks the
 // tile as runs at one point (the pair geometry hoisted once per run);
@@ -950,7 +954,7 @@ __device__ __forceinline__ unsigned long long tile_members(const LaneClasses& lc
     bool pend = lane < m;
     unsigned long long mm = 0ull, an = 0ull;
 #pragma unroll
-    for (int k = 0; k < kP0Classes; ++k) {
+    for (int k = 0; k < kCandPoints; ++k) {
       const unsigned long long pm = __ballot(pend);
       if (pm == 0ull) break;
       const float3 pk = bcast(t0, __builtin_ctzll(pm));
