@@ -83,7 +83,7 @@ int main(int argc, char** argv) {
       ++fails;
     }
     // the in-place wrapper, on both sides of its cut-over
-    const int ks = 1 + (k % 48);
+    const int ks = 1 + (k % (2 * fluidhip::kAddInPlace));
     if (bits(fluidhip::add_k(x, w, ks)) != bits(naive(x, w, ks)) ||
         bits(fluidhip::add_k(x, w, k)) != bits(b)) {
       if (fails < 10) std::printf("MISMATCH add_k x=%a w=%a k=%d\n", x, w, k);
