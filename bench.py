@@ -107,6 +107,69 @@ def copy_calibration(mbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": n}))
 
 
+def summarize_traffic(outdir: str, steps: int, json_path: str) -> str:
+    """HBM bytes per launch of lambda and delta from four `rocprofv3 --pmc`
+    runs under `outdir` (pmc_fetch: FETCH_SIZE and pmc_write: WRITE_SIZE of
+    this script's headline workload; pmc_cfetch / pmc_cwrite: the same
+    counters for `--copy-calibration 256`).  Each kernel counter (KiB per
+    dispatch) is scaled by what it reads for the copy of known size and
+    averaged over the last 4 * `steps` dispatches in dispatch order.  Writes
+    the JSON that `measured_traffic` reads and returns a text table."""
+    import collections
+    import glob
+    import sqlite3
+
+    def per_dispatch(sub, counter):
+        dbs = sorted(glob.glob(os.path.join(outdir, sub, "**", "*_results.db"), recursive=True))
+        if not dbs:
+            raise SystemExit(f"no rocpd database under {os.path.join(outdir, sub)}")
+        rows = sqlite3.connect(dbs[0]).cursor().execute(
+            "select dispatch_id, name, sum(counter_value) from pmc_events where counter_name=? "
+            "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+        out = collections.defaultdict(list)
+        for did, name, v in rows:
+            out[name].append((did, v))
+        return out
+
+    copy_kib = 256 * 1024
+
+    def copy_reading(d):
+        vals = [v for k in d if "copyBuffer" in k for _, v in d[k]]
+        return sum(vals) / len(vals)
+
+    f_read = copy_reading(per_dispatch("pmc_cfetch", "FETCH_SIZE"))
+    w_read = copy_reading(per_dispatch("pmc_cwrite", "WRITE_SIZE"))
+    f_scale, w_scale = copy_kib / f_read, copy_kib / w_read
+    kf = per_dispatch("pmc_fetch", "FETCH_SIZE")
+    kw = per_dispatch("pmc_write", "WRITE_SIZE")
+    n = 1 << 20
+    alg = {"lambda": 16.0 * n, "delta": 28.0 * n}
+    res = {"workload": "million_block 128x128x64 all flags, 4 iters; substeps 31-33 "
+                       "(the first substeps of the timed window)",
+           "copy_calibration": {"bytes": copy_kib * 1024, "fetch_kib_read": f_read,
+                                "write_kib_read": w_read, "fetch_scale": f_scale,
+                                "write_scale": w_scale},
+           "kernels": {}}
+    lines = [f"Calibration: FETCH_SIZE {f_read:.0f} KiB -> scale {f_scale:.4f}, "
+             f"WRITE_SIZE {w_read:.0f} KiB -> scale {w_scale:.4f}", "",
+             "kernel       fetch MB   write MB     HBM MB  algorithmic MB   ratio"]
+    for key in ("lambda", "delta"):
+        fv = [v for _, v in sorted(x for k in kf if f"k_{key}" in k for x in kf[k])][-4 * steps:]
+        wv = [v for _, v in sorted(x for k in kw if f"k_{key}" in k for x in kw[k])][-4 * steps:]
+        fetch = sum(fv) / len(fv) * 1024 * f_scale
+        write = sum(wv) / len(wv) * 1024 * w_scale
+        res["kernels"][key] = {"fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+                               "hbm_bytes_per_launch": fetch + write,
+                               "algorithmic_bytes_per_launch": alg[key],
+                               "traffic_over_algorithmic": (fetch + write) / alg[key],
+                               "launches": len(fv)}
+        lines.append(f"{key:<10}{fetch / 1e6:11.1f}{write / 1e6:11.1f}{(fetch + write) / 1e6:11.1f}"
+                     f"{alg[key] / 1e6:16.1f}{(fetch + write) / alg[key]:8.1f}")
+    with open(json_path, "w") as f:
+        json.dump(res, f, indent=1)
+    return "\n".join(lines)
+
+
 def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
     """Text summary of a `rocprofv3 --kernel-trace --stats` run of this
     script (rocpd database output): per-kernel totals over every dispatch of
@@ -360,6 +423,11 @@ def main():
                     help="only run a device copy of this size (counter calibration)")
     ap.add_argument("--summarize-rocprof", type=str, default="", metavar="DIR",
                     help="print a summary of a rocprofv3 output directory of this script")
+    ap.add_argument("--summarize-traffic", type=str, default="", metavar="DIR",
+                    help="HBM traffic per launch from the pmc_fetch / pmc_write / pmc_cfetch / "
+                         "pmc_cwrite rocprofv3 runs under DIR (writes --traffic-json)")
+    ap.add_argument("--traffic-json", type=str,
+                    default=os.path.join(ROOT, "profiles", "hbm_traffic_million_block.json"))
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"million_block": 120, "ensemble": 240, "adjoint": 100}[args.workload]
@@ -367,6 +435,9 @@ def main():
         args.warmup = 30 if args.workload == "million_block" else 0
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.summarize_traffic:
+        print(summarize_traffic(args.summarize_traffic, args.steps, args.traffic_json))
         return
     if args.summarize_rocprof:
         print(summarize_rocprof(args.summarize_rocprof, args.steps,
