@@ -1,0 +1,83 @@
+"""The measurement tools of bench.py on synthetic rocprofv3 (rocpd) databases:
+the HBM-traffic summary scales each counter by its reading for a copy of known
+size and averages the last dispatches in dispatch order, across template
+variants of one kernel; the kernel-trace window leaves out dispatches that
+returned at once after a capacity overflow."""
+import json
+import os
+import sqlite3
+
+import bench
+
+
+def _pmc_db(path, rows):
+    """rows: (dispatch_id, kernel name, counter, value)."""
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    con = sqlite3.connect(path)
+    con.execute("create table pmc_events (dispatch_id int, name text, counter_name text, "
+                "counter_value real)")
+    con.executemany("insert into pmc_events values (?, ?, ?, ?)", rows)
+    con.commit()
+    con.close()
+
+
+def _kernel_db(path, rows):
+    """rows: (kernel name, start, duration)."""
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    con = sqlite3.connect(path)
+    con.execute("create table kernels (name text, start int, duration int)")
+    con.executemany("insert into kernels values (?, ?, ?)", rows)
+    con.commit()
+    con.close()
+
+
+def test_traffic_summary_scales_and_orders_dispatches(tmp_path):
+    copy_kib = 256 * 1024
+    # the copy reads half its bytes through FETCH_SIZE: scale 2
+    _pmc_db(str(tmp_path / "pmc_cfetch" / "c_results.db"),
+            [(1, "__amd_rocclr_copyBuffer", "FETCH_SIZE", copy_kib / 2)] * 1)
+    _pmc_db(str(tmp_path / "pmc_cwrite" / "c_results.db"),
+            [(1, "__amd_rocclr_copyBuffer", "WRITE_SIZE", copy_kib)])
+    # delta: iterations 0-2 under one template name, the last under another;
+    # the early dispatches (warm-up) read 1000 KiB, the last three substeps 10
+    fetch, write = [], []
+    did = 0
+    for step in range(6):
+        for it in range(4):
+            did += 1
+            name = "k_delta<true, true, false>" if it == 3 else "k_delta<true, false, false>"
+            v = 1000.0 if step < 3 else 10.0
+            fetch += [(did, name, "FETCH_SIZE", v)]
+            write += [(did, name, "WRITE_SIZE", v / 2)]
+            did += 1
+            fetch += [(did, "k_lambda", "FETCH_SIZE", v)]
+            write += [(did, "k_lambda", "WRITE_SIZE", v / 2)]
+    _pmc_db(str(tmp_path / "pmc_fetch" / "f_results.db"), fetch)
+    _pmc_db(str(tmp_path / "pmc_write" / "w_results.db"), write)
+    out = str(tmp_path / "traffic.json")
+    text = bench.summarize_traffic(str(tmp_path), 3, out)
+    d = json.load(open(out))
+    assert abs(d["copy_calibration"]["fetch_scale"] - 2.0) < 1e-12
+    for k in ("lambda", "delta"):
+        r = d["kernels"][k]
+        assert r["launches"] == 12
+        # only the last three substeps (10 KiB each) count, both template variants
+        assert abs(r["fetch_bytes_per_launch"] - 10.0 * 1024 * 2.0) < 1e-6
+        assert abs(r["write_bytes_per_launch"] - 5.0 * 1024) < 1e-6
+    assert "lambda" in text and "delta" in text
+
+
+def test_rocprof_window_skips_instant_dispatches(tmp_path):
+    rows, t = [], 0
+    # 5 real substeps of lambda at 2000 us, then a capacity overflow: 2
+    # substeps queued behind it return at once (2 us), then 3 real again
+    for us in [2000.0] * 5 + [2.0] * 2 + [2000.0] * 3:
+        for _ in range(4):
+            t += 1
+            rows.append(("k_lambda", t, int(us * 1000)))
+    _kernel_db(str(tmp_path / "kt" / "h_results.db"), rows)
+    text = bench.summarize_rocprof(str(tmp_path / "kt"), 6, {"lambda": 4})
+    line = [l for l in text.splitlines() if l.strip().startswith("lambda")][0]
+    assert "launches     24" in line
+    assert "avg_us       2000.0" in line
+    assert "(skipped 8)" in line
