@@ -1164,9 +1164,10 @@ struct TileP0 {
 //    the tile is walked as maximal runs of consecutive candidates at one
 //    point cp: for each run prep(cp, flag) hoists the pair geometry once
 //    per lane and says how the lane's neighbors in the run contribute:
-//    0 one by one through pair_u, 1 nothing at all (exact signed zeros),
-//    2 `count` identical terms (on_count); flag is tflag(tile, m), evaluated
-//    by the whole wavefront;
+//    0 one by one through pair_u (or, when the run's candidates hold one
+//    record, that one term `count` times exactly through pair_rep),
+//    1 nothing at all (exact signed zeros), 2 `count` identical terms
+//    (on_count); flag is tflag(tile, m), evaluated by the whole wavefront;
 //  * consecutive tiles whose candidates all hold one record (bit for bit,
 //    p^0 included) form one range: each lane only counts its members, and
 //    the range then contributes like one run at one point, with mode 0
