@@ -1595,9 +1595,11 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
 // also commits (core.cpp:414-421) into posA/vout; kSolve=false is the
 // zero-iteration commit.  kRest fuses plane restitution (core.cpp:573-612).
 // A recorded substep also stores the planes that pushed (pmask) and the
-// restitution decisions (rmask).
+// restitution decisions (rmask).  Held to 96 VGPRs (5 wavefronts per SIMD):
+// measured 1.4% faster than the compiler's 107 (XSPH below likewise, 0.8%;
+// λ would spill to scratch, ω measured slower).
 template <bool kSolve, bool kLast, bool kRest>
-__global__ __launch_bounds__(1024) void k_delta(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void k_delta(
     StepConsts c, const float4* __restrict__ P, float4* __restrict__ Pnext, NbrView nv,
     CellView cv, const ScenePlanes* __restrict__ sp, const float4* __restrict__ posB,
     float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
@@ -1719,7 +1721,7 @@ __global__ __launch_bounds__(1024) void k_delta(
 
 // XSPH (core.cpp:430-465); vin.w carries rho_j of the last lambda pass.
 template <bool kRest>
-__global__ __launch_bounds__(1024) void k_xsph(StepConsts c, const float4* __restrict__ pos,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void k_xsph(StepConsts c, const float4* __restrict__ pos,
                                                const float4* __restrict__ vin,
                                                float4* __restrict__ vout, NbrView nv,
                                                CellView cv, const ScenePlanes* __restrict__ sp,
