@@ -246,8 +246,10 @@ int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel,
 
 /* Per-kernel device time (HIP events on the context's stream) of the steps
  * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_count",
- * "scan", "scatter", "reorder", "neighbors", "lambda", "delta", "xsph",
- * "vorticity_omega", "vorticity_apply", "finalize". */
+ * "scan", "sort", "reorder", "runs", "neighbors", "lambda", "delta", "xsph",
+ * "vorticity_omega", "vorticity_apply".  Launches of a substep that
+ * overflowed a capacity, and of the substeps queued behind it, are not
+ * counted (fluid_hip_step runs them again after growing the tables). */
 int fluid_hip_profile(fluid_hip_ctx* ctx, int32_t enable);
 int fluid_hip_profile_read(fluid_hip_ctx* ctx, const char* kernel, double* total_ms,
                            int64_t* launches);
