@@ -81,3 +81,16 @@ def test_rocprof_window_skips_instant_dispatches(tmp_path):
     assert "launches     24" in line
     assert "avg_us       2000.0" in line
     assert "(skipped 8)" in line
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    """bench.py measures one GPU; asking for more exits with status 2 and
+    says why, before touching any device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(os.path.dirname(bench.__file__), "bench.py"),
+                        "--gpus", "2", "--steps", "1", "--warmup", "0", "--no-cpu"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
