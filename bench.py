@@ -40,6 +40,12 @@ KERNEL_BYTES = {
     "vorticity_apply": 52.0,   # pos 12 + omega 12 + |omega| 4 + vel 12 -> vel 12
 }
 STEP_BYTES_ALL_FLAGS = 480.0   # whole substep, all flags, 4 iterations
+# Algorithmic FP32 operations per neighbor pair (add, mul, div, sqrt and
+# compare-select each count one; loop invariants hoisted): lambda 33, delta
+# with s_corr 30, XSPH 25, omega 31, eta and confinement 23.
+FLOPS_PER_PAIR = {"lambda": 33.0, "delta": 30.0, "xsph": 25.0, "vorticity_omega": 31.0,
+                  "vorticity_apply": 23.0}
+FP32_PEAK_TFLOPS = 157.3
 KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "runs",
            "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply"]
 
@@ -52,40 +58,86 @@ def sync_all():
         torch.cuda.synchronize()
 
 
-def measured_traffic(kernel: str, dims) -> float | None:
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
-    same counter's reading for a device copy of known size), if present."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_million_block.json")
-    if tuple(dims) != (128, 128, 64) or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        data = json.load(f)
-    k = data.get("kernels", {}).get(kernel)
-    return None if k is None else float(k["hbm_bytes_per_launch"])
+TRAFFIC_GLOB = "traffic_million_block_*.json"
 
 
-def cpu_baseline(state, params, threads, steps=1, warm=0):
-    """The reference's own CPU step (oracle/_ref, built from core.cpp) on the
-    state the GPU timed region starts from."""
+def measured_traffic(kernel: str, run: dict):
+    """HBM bytes per launch and L2 hit rate of `kernel` from a committed
+    rocprofv3 counter summary (profiles/traffic_million_block_*.json) whose
+    workload, window (warm-up, timed substeps) and launch configuration are
+    this run's.  Returns (bytes or None, hit rate or None, note)."""
+    import glob
+
+    wanted = {"dims": list(run["dims"]), "warmup": run["warmup"], "steps": run["steps"],
+              "config": run["config"]}
+    seen = []
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", TRAFFIC_GLOB))):
+        with open(path) as f:
+            data = json.load(f)
+        rec = data.get("run", {})
+        got = {"dims": rec.get("dims"), "warmup": rec.get("warmup"), "steps": rec.get("steps"),
+               "config": rec.get("config")}
+        seen.append(os.path.basename(path))
+        if got != wanted:
+            continue
+        k = data.get("kernels", {}).get(kernel)
+        if k is None:
+            return None, None, f"{os.path.basename(path)} has no {kernel} record"
+        return (float(k["hbm_bytes_per_launch"]), k.get("l2_hit_rate"),
+                f"profiles/{os.path.basename(path)}")
+    return None, None, ("no committed counter summary for this window and launch "
+                        f"configuration {wanted} (have: {', '.join(seen) or 'none'})")
+
+
+def cpu_threads() -> int:
+    """Host threads of the CPU baseline: OMP_NUM_THREADS when set (the share
+    of the host that belongs to this GPU), else every core up to 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(state, params, threads, warm=1, steps=2, one_thread=True):
+    """The reference's own CPU step (oracle/_ref, built from core.cpp), or the
+    oracle where that library is absent, on the state the GPU timed region
+    starts from: `warm` untimed steps, then `steps` timed steps on `threads`
+    OpenMP threads (one timed step when a step takes over 15 s), and one
+    timed step on one thread from the same state."""
     import oracle
 
     which = "ref" if oracle.have_ref() else "oracle"
     oracle.set_threads(threads, which)
+    t0 = time.perf_counter()
     r = oracle.step(params, state.pos, state.vel, nsteps=warm, which=which)
+    t_warm = (time.perf_counter() - t0) / max(warm, 1)
+    if t_warm > 15.0:
+        steps = 1
     t0 = time.perf_counter()
     oracle.step(params, r["pos"], r["vel"], nsteps=steps, which=which)
     dt = time.perf_counter() - t0
-    return {
+    kind = "reference" if which == "ref" else "port"
+    name = "reference core.cpp" if which == "ref" else "oracle"
+    out = {
         "value": steps / dt,
         "unit": "steps/s",
         "cores": threads,
-        "kind": "reference" if which == "ref" else "port",
-        "sample": (f"{'reference core.cpp' if which == 'ref' else 'oracle'} step() on the "
-                   f"{state.n}-particle state the GPU timed region starts from: {warm} untimed "
-                   f"+ {steps} timed steps, {threads} OpenMP threads "
+        "kind": kind,
+        "sample": (f"{name} step() on the {state.n}-particle state the GPU timed region starts "
+                   f"from: {warm} untimed + {steps} timed steps on {threads} OpenMP threads "
                    f"({steps * state.n / dt:.3g} particle-steps/s)"),
+        "particle_steps_per_s": steps * state.n / dt,
     }
+    if one_thread:
+        oracle.set_threads(1, which)
+        t0 = time.perf_counter()
+        oracle.step(params, r["pos"], r["vel"], nsteps=1, which=which)
+        d1 = time.perf_counter() - t0
+        oracle.set_threads(threads, which)
+        out["one_thread"] = {"value": 1.0 / d1, "unit": "steps/s", "cores": 1,
+                             "sample": "1 timed step on 1 thread from the same state",
+                             "particle_steps_per_s": state.n / d1}
+    return out
 
 
 def copy_calibration(mbytes: int) -> None:
@@ -107,22 +159,38 @@ def copy_calibration(mbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": n}))
 
 
-def summarize_traffic(outdir: str, steps: int, json_path: str) -> str:
-    """HBM bytes per launch of lambda and delta from four `rocprofv3 --pmc`
-    runs under `outdir` (pmc_fetch: FETCH_SIZE and pmc_write: WRITE_SIZE of
-    this script's headline workload; pmc_cfetch / pmc_cwrite: the same
-    counters for `--copy-calibration 256`).  Each kernel counter (KiB per
-    dispatch) is scaled by what it reads for the copy of known size and
-    averaged over the last 4 * `steps` dispatches in dispatch order.  Writes
-    the JSON that `measured_traffic` reads and returns a text table."""
+# rocprofv3 kernel names of the roofline kernels, and their launches per
+# substep (lambda and delta: one per constraint iteration)
+ROCPROF_NAME = {"predict_bbox": "k_predict_bbox", "lambda": "k_lambda", "delta": "k_delta",
+                "xsph": "k_xsph", "vorticity_omega": "k_vort_omega",
+                "vorticity_apply": "k_vort_apply"}
+
+
+def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | None = None,
+                      iterations: int = 4) -> str:
+    """HBM bytes per launch and L2 hit rate of the roofline kernels from
+    `rocprofv3 --pmc` runs under `outdir`: pmc_fetch (FETCH_SIZE) and
+    pmc_write (WRITE_SIZE) of one bench command, pmc_l2 (TCC_HIT_sum,
+    TCC_MISS_sum) of the same command when present, and pmc_cfetch /
+    pmc_cwrite (the same counters for `--copy-calibration 256`).  Each
+    traffic counter (KiB per dispatch) is scaled by what it reads for the copy
+    of known size; every figure is averaged over the kernel's dispatches of
+    the timed window (the last `steps` substeps in dispatch order).  The bench
+    line the fetch run printed (pmc_fetch/bench.json) supplies the workload,
+    window and launch configuration, which the JSON records so that
+    `measured_traffic` only uses it for a run of the same kind.  Writes the
+    JSON (default profiles/traffic_million_block_warmup<W>_steps<K>.json)
+    and returns a text table."""
     import collections
     import glob
     import sqlite3
 
-    def per_dispatch(sub, counter):
+    def per_dispatch(sub, counter, required=True):
         dbs = sorted(glob.glob(os.path.join(outdir, sub, "**", "*_results.db"), recursive=True))
         if not dbs:
-            raise SystemExit(f"no rocpd database under {os.path.join(outdir, sub)}")
+            if required:
+                raise SystemExit(f"no rocpd database under {os.path.join(outdir, sub)}")
+            return None
         rows = sqlite3.connect(dbs[0]).cursor().execute(
             "select dispatch_id, name, sum(counter_value) from pmc_events where counter_name=? "
             "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
@@ -130,6 +198,22 @@ def summarize_traffic(outdir: str, steps: int, json_path: str) -> str:
         for did, name, v in rows:
             out[name].append((did, v))
         return out
+
+    run = {}
+    line_path = os.path.join(outdir, "pmc_fetch", "bench.json")
+    if os.path.exists(line_path):
+        with open(line_path) as f:
+            line = json.loads(f.read().strip().splitlines()[-1])
+        cfg = line.get("config", {})
+        run = {"dims": cfg.get("dims"), "warmup": line.get("warmup"), "steps": line.get("steps"),
+               "config": {k: cfg.get(k) for k in ("block", "xcd_remap", "wave_mode",
+                                                  "brick_edge", "graph")}}
+        if steps is None:
+            steps = int(line["steps"])
+    if steps is None:
+        raise SystemExit("give --steps or a pmc_fetch/bench.json bench line")
+    dims = run.get("dims") or [128, 128, 64]
+    n = int(np.prod(dims))
 
     copy_kib = 256 * 1024
 
@@ -142,29 +226,47 @@ def summarize_traffic(outdir: str, steps: int, json_path: str) -> str:
     f_scale, w_scale = copy_kib / f_read, copy_kib / w_read
     kf = per_dispatch("pmc_fetch", "FETCH_SIZE")
     kw = per_dispatch("pmc_write", "WRITE_SIZE")
-    n = 1 << 20
-    alg = {"lambda": 16.0 * n, "delta": 28.0 * n}
-    res = {"workload": "million_block 128x128x64 all flags, 4 iters; substeps 31-33 "
-                       "(the first substeps of the timed window)",
+    kh = per_dispatch("pmc_l2", "TCC_HIT_sum", required=False)
+    km = per_dispatch("pmc_l2", "TCC_MISS_sum", required=False)
+
+    def window(d, key, per_step):
+        if d is None:
+            return []
+        name = ROCPROF_NAME[key]
+        vals = [v for _, v in sorted(x for k in d if name in k for x in d[k])]
+        return vals[-per_step * steps:]
+
+    res = {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]}, the timed window of "
+                       "the bench command",
+           "run": run,
            "copy_calibration": {"bytes": copy_kib * 1024, "fetch_kib_read": f_read,
                                 "write_kib_read": w_read, "fetch_scale": f_scale,
                                 "write_scale": w_scale},
            "kernels": {}}
     lines = [f"Calibration: FETCH_SIZE {f_read:.0f} KiB -> scale {f_scale:.4f}, "
              f"WRITE_SIZE {w_read:.0f} KiB -> scale {w_scale:.4f}", "",
-             "kernel       fetch MB   write MB     HBM MB  algorithmic MB   ratio"]
-    for key in ("lambda", "delta"):
-        fv = [v for _, v in sorted(x for k in kf if f"k_{key}" in k for x in kf[k])][-4 * steps:]
-        wv = [v for _, v in sorted(x for k in kw if f"k_{key}" in k for x in kw[k])][-4 * steps:]
+             "kernel              fetch MB   write MB     HBM MB  algorithmic MB   ratio  L2 hit"]
+    for key, per_particle in KERNEL_BYTES.items():
+        per_step = iterations if key in ("lambda", "delta") else 1
+        fv, wv = window(kf, key, per_step), window(kw, key, per_step)
+        if not fv or not wv:
+            continue
         fetch = sum(fv) / len(fv) * 1024 * f_scale
         write = sum(wv) / len(wv) * 1024 * w_scale
+        hv, mv = window(kh, key, per_step), window(km, key, per_step)
+        hit = (sum(hv) / (sum(hv) + sum(mv))) if hv and mv and (sum(hv) + sum(mv)) > 0 else None
+        alg = per_particle * n
         res["kernels"][key] = {"fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
                                "hbm_bytes_per_launch": fetch + write,
-                               "algorithmic_bytes_per_launch": alg[key],
-                               "traffic_over_algorithmic": (fetch + write) / alg[key],
-                               "launches": len(fv)}
-        lines.append(f"{key:<10}{fetch / 1e6:11.1f}{write / 1e6:11.1f}{(fetch + write) / 1e6:11.1f}"
-                     f"{alg[key] / 1e6:16.1f}{(fetch + write) / alg[key]:8.1f}")
+                               "algorithmic_bytes_per_launch": alg,
+                               "traffic_over_algorithmic": (fetch + write) / alg,
+                               "l2_hit_rate": hit, "launches": len(fv)}
+        lines.append(f"{key:<18}{fetch / 1e6:11.1f}{write / 1e6:11.1f}{(fetch + write) / 1e6:11.1f}"
+                     f"{alg / 1e6:16.1f}{(fetch + write) / alg:8.2f}"
+                     f"{'' if hit is None else f'{hit:8.3f}'}")
+    if json_path is None:
+        json_path = os.path.join(ROOT, "profiles", f"traffic_million_block_warmup"
+                                 f"{run.get('warmup')}_steps{steps}.json")
     with open(json_path, "w") as f:
         json.dump(res, f, indent=1)
     return "\n".join(lines)
@@ -395,6 +497,41 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def read_kernels(s) -> dict:
+    """Per-kernel device time of the substeps queued since profiling was
+    switched on (HIP events on the context's stream)."""
+    out = {}
+    for k in KERNELS:
+        ms, cnt = s.profile_read(k)
+        if cnt:
+            out[k] = {"total_ms": ms, "launches": cnt, "avg_us": 1000.0 * ms / cnt}
+    return out
+
+
+def roofline_record(kernels: dict, n: int, mean_neighbors: float) -> dict:
+    """Roofline of the kernel with the most device time among the gather
+    passes: algorithmic bytes per launch (KERNEL_BYTES x n) over its average
+    launch time, against the HBM peak; the FP32 fraction from the algorithmic
+    flops per neighbor pair (FLOPS_PER_PAIR x mean neighbors x n).  Adds
+    both figures to every kernel's entry."""
+    for k, v in kernels.items():
+        if k in KERNEL_BYTES:
+            t = v["avg_us"] * 1e-6
+            v["algorithmic_GBs"] = KERNEL_BYTES[k] * n / t / 1e9
+            v["hbm_frac"] = v["algorithmic_GBs"] / HBM_PEAK_GBS
+            if k in FLOPS_PER_PAIR:
+                v["algorithmic_TFLOPs"] = FLOPS_PER_PAIR[k] * mean_neighbors * n / t / 1e12
+                v["fp32_frac"] = v["algorithmic_TFLOPs"] / FP32_PEAK_TFLOPS
+    ranked = sorted((k for k in kernels if k in KERNEL_BYTES), key=lambda k: -kernels[k]["total_ms"])
+    dom = ranked[0]
+    v = kernels[dom]
+    return {"bound": "hbm", "kernel": dom, "achieved": v["algorithmic_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": v["hbm_frac"], "traffic": None,
+            "bytes_per_launch": KERNEL_BYTES[dom] * n,
+            "fp32_TFLOPs": v.get("algorithmic_TFLOPs"), "fp32_peak_TFLOPs": FP32_PEAK_TFLOPS,
+            "fp32_frac": v.get("fp32_frac")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -404,6 +541,8 @@ def main():
                     help="untimed substeps first (default: 30 million_block, 0 ensemble)")
     ap.add_argument("--graph", action="store_true", help="replay a captured hipGraph per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu-1t", action="store_true",
+                    help="skip the one-thread step of the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--xcd-remap", type=int, default=1)
@@ -424,10 +563,12 @@ def main():
     ap.add_argument("--summarize-rocprof", type=str, default="", metavar="DIR",
                     help="print a summary of a rocprofv3 output directory of this script")
     ap.add_argument("--summarize-traffic", type=str, default="", metavar="DIR",
-                    help="HBM traffic per launch from the pmc_fetch / pmc_write / pmc_cfetch / "
-                         "pmc_cwrite rocprofv3 runs under DIR (writes --traffic-json)")
-    ap.add_argument("--traffic-json", type=str,
-                    default=os.path.join(ROOT, "profiles", "hbm_traffic_million_block.json"))
+                    help="HBM traffic and L2 hit rate per launch from the pmc_fetch / pmc_write "
+                         "/ pmc_l2 / pmc_cfetch / pmc_cwrite rocprofv3 runs under DIR (writes "
+                         "--traffic-json)")
+    ap.add_argument("--traffic-json", type=str, default="",
+                    help="output of --summarize-traffic (default: "
+                         "profiles/traffic_million_block_warmup<W>_steps<K>.json)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"million_block": 120, "ensemble": 240, "adjoint": 100}[args.workload]
@@ -437,7 +578,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.summarize_traffic:
-        print(summarize_traffic(args.summarize_traffic, args.steps, args.traffic_json))
+        print(summarize_traffic(args.summarize_traffic, args.traffic_json or None))
         return
     if args.summarize_rocprof:
         print(summarize_rocprof(args.summarize_rocprof, args.steps,
@@ -455,19 +596,35 @@ def main():
         print(json.dumps(bench_adjoint(args)))
         return
 
-    dims =tuple(int(x) for x in args.dims.split(","))
+    dims = tuple(int(x) for x in args.dims.split(","))
     sc = scenes.million_block(dims=dims)
     n = sc.state.n
     s = hip.HipSolver(sc.params, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap),
                       brick_edge=args.brick_edge)
     s.upload(sc.state.pos, sc.state.vel, 0.0)
-    s.step(args.warmup, graph=args.graph)
+    prof = not args.no_profile and not args.graph
+    early = None
+    warm_left = args.warmup
+    if prof and args.warmup >= 3:
+        # substeps 2-3 of the warm-up, before particles pile up (about 28
+        # neighbors each): the regime the algorithmic byte model describes
+        s.step(1)
+        s.profile(True)
+        s.step(2)
+        early_k = read_kernels(s)
+        s.profile(False)
+        pairs = s.stats()["pairs"]
+        early = roofline_record(early_k, n, mean_neighbors=pairs / n)
+        early["substeps"] = "2-3"
+        early["mean_neighbors"] = pairs / n
+        early["kernels"] = early_k
+        warm_left -= 3
+    s.step(warm_left, graph=args.graph)
     start_state = s.download() if not args.no_cpu else None
     # the tuner's trials start from this state and leave it in place (they
     # also size the tables for it); the timed region starts here too
     tuned = s.autotune(args.autotune) if args.autotune > 0 else None
     s.sync()
-    prof = not args.no_profile and not args.graph
     if prof:
         s.profile(True)
     sync_all()
@@ -477,38 +634,35 @@ def main():
     sync_all()
     elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed
+    kernels = read_kernels(s) if prof else {}
+    if prof:
+        s.profile(False)
     st = s.stats()
     sizes = {"neighbor_pairs_last_step": st["pairs"], "list_words_last_step": st["words"],
              "grid_cells_last_step": st["cells"],
              "capacity_growths_in_window": st["grow_events"] - grows_before}
-    kernels = {}
-    if prof:
-        for k in KERNELS:
-            ms, cnt = s.profile_read(k)
-            if cnt:
-                kernels[k] = {"total_ms": ms, "launches": cnt, "avg_us": 1000.0 * ms / cnt}
-        s.profile(False)
     s.close()
 
+    config = {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]} all flags, 4 iters",
+              "dims": list(dims), "particles": n, "graph": bool(args.graph),
+              "block": tuned["block_size"] if tuned else args.block,
+              "xcd_remap": bool(tuned["xcd_remap"] if tuned else args.xcd_remap),
+              "wave_mode": tuned["wave_mode"] if tuned else 2,
+              "brick_edge": tuned["brick_edge"] if tuned else (args.brick_edge or 4)}
     roofline = None
     if kernels:
-        ranked = sorted((k for k in kernels if k in KERNEL_BYTES),
-                        key=lambda k: -kernels[k]["total_ms"])
-        dom = ranked[0]
-        avg_s = kernels[dom]["avg_us"] * 1e-6
-        achieved = KERNEL_BYTES[dom] * n / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": measured_traffic(dom, dims),
-                    "bytes_per_launch": KERNEL_BYTES[dom] * n}
-        for k, v in kernels.items():
-            if k in KERNEL_BYTES:
-                v["algorithmic_GBs"] = KERNEL_BYTES[k] * n / (v["avg_us"] * 1e-6) / 1e9
+        roofline = roofline_record(kernels, n, mean_neighbors=st["pairs"] / max(n, 1))
+        run = {"dims": list(dims), "warmup": args.warmup, "steps": args.steps,
+               "config": {k: config[k] for k in ("block", "xcd_remap", "wave_mode", "brick_edge",
+                                                 "graph")}}
+        traffic, l2, note = measured_traffic(roofline["kernel"], run)
+        roofline["traffic"] = traffic
+        roofline["l2_hit_rate"] = l2
+        roofline["traffic_source"] = note
 
     cpu = None
     if start_state is not None:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(start_state, sc.params, threads)
+        cpu = cpu_baseline(start_state, sc.params, cpu_threads(), one_thread=not args.no_cpu_1t)
 
     step_gbs = STEP_BYTES_ALL_FLAGS * n * steps_per_s / 1e9
     out = {
@@ -524,16 +678,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice)",
-        "config": {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]} all flags, 4 iters",
-                   "particles": n, "graph": bool(args.graph),
-                   "block": tuned["block_size"] if tuned else args.block,
-                   "xcd_remap": bool(tuned["xcd_remap"] if tuned else args.xcd_remap),
-                   "wave_mode": tuned["wave_mode"] if tuned else 2,
-                   "brick_edge": tuned["brick_edge"] if tuned else (args.brick_edge or 4)},
+        "config": config,
         "autotune": tuned,
         "particle_steps_per_s": steps_per_s * n,
         "step_hbm_fraction": step_gbs / HBM_PEAK_GBS,
         "roofline": roofline,
+        "roofline_early": early,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "sizes": sizes,

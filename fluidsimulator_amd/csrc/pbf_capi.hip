@@ -290,6 +290,10 @@ int collect_profile(Ctx& c, int keep_steps = INT_MAX) {
 int queue_steps(Ctx& c, int nsteps, bool use_graph) {
   if (use_graph && c.n > 0) {
     if (!c.graph_exec) {
+      // The graph holds every buffer address of one substep: it is dropped
+      // (drop_graph) whenever a buffer is reallocated or a launch knob,
+      // parameter or stream changes, and captured again here.
+      drop_graph(c);
       HIP_TRY(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal),
               "hipStreamBeginCapture");
       const bool prof = c.prof.on;
@@ -298,14 +302,19 @@ int queue_steps(Ctx& c, int nsteps, bool use_graph) {
       c.prof.on = prof;
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(c.stream, &g);
-      if (e != hipSuccess) {
+      if (e != hipSuccess || e2 != hipSuccess) {
         if (g) hipGraphDestroy(g);
-        return hip_fail(e, "substep capture");
+        return e != hipSuccess ? hip_fail(e, "substep capture")
+                               : hip_fail(e2, "hipStreamEndCapture");
       }
-      if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+      hipGraphExec_t x = nullptr;
+      const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e3 != hipSuccess) {
+        hipGraphDestroy(g);  // nothing is left half-built for the next call
+        return hip_fail(e3, "hipGraphInstantiate");
+      }
       c.graph = g;
-      HIP_TRY(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0),
-              "hipGraphInstantiate");
+      c.graph_exec = x;
     }
     for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(c.graph_exec, c.stream), "hipGraphLaunch");
   } else {
@@ -870,6 +879,12 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
     }
   }
   if (nbr_count) {
+    if (c.lists_partial) {
+      // particles of dense cells and non-head slots of runs kept no count:
+      // a counting walk of the last grid writes every slot's
+      HIP_TRY(fluidhip::enqueue_count_pairs(c), "count pairs");
+      HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+    }
     std::vector<int> nc(n);
     HIP_TRY(hipMemcpy(nc.data(), c.ncount, sizeof(int) * n, hipMemcpyDeviceToHost), "read counts");
     for (size_t s = 0; s < n; ++s) nbr_count[ids[s]] = nc[s];
@@ -877,7 +892,20 @@ int fluid_hip_get_diag(fluid_hip_ctx* ctx, int32_t* cell, int32_t* sorted_ids,
   if (rho) {
     std::vector<float> r(n);
     HIP_TRY(hipMemcpy(r.data(), c.rho, sizeof(float) * n, hipMemcpyDeviceToHost), "read rho");
-    for (size_t s = 0; s < n; ++s) rho[ids[s]] = c.iterations > 0 ? r[s] : 0.0f;
+    // with runs only each run's first slot wrote rho; the others equal it
+    std::vector<int> run_no, rep;
+    if (c.last_runs) {
+      run_no.resize(n);
+      rep.resize(n);
+      HIP_TRY(hipMemcpy(run_no.data(), c.run_no, sizeof(int) * n, hipMemcpyDeviceToHost),
+              "read runs");
+      HIP_TRY(hipMemcpy(rep.data(), c.rep_list, sizeof(int) * n, hipMemcpyDeviceToHost),
+              "read runs");
+    }
+    for (size_t s = 0; s < n; ++s) {
+      const size_t src = c.last_runs ? static_cast<size_t>(rep[run_no[s]]) : s;
+      rho[ids[s]] = c.iterations > 0 ? r[src] : 0.0f;
+    }
   }
   if (lambda) {
     if (c.iterations > 0) {

@@ -124,6 +124,9 @@ struct Ctx {
   // the last substep stored no lists for cooperative particles (their
   // neighbor counts need a counting walk), and its p^0 is in pred[0]
   bool lists_partial = false;
+  // the last substep computed one lane per run of identical slots (the
+  // run's other slots copied its outputs; rho was written by the heads only)
+  bool last_runs = false;
 
   // staging for transfers
   float* stage = nullptr;  // 6 * n

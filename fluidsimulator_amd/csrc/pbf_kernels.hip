@@ -511,12 +511,15 @@ __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
 __global__ __launch_bounds__(1024) void k_sort_keys(StepConsts c, const float4* __restrict__ pos,
                                                     const int* __restrict__ key,
                                                     int* __restrict__ kout, int* __restrict__ vout,
-                                                    const unsigned* __restrict__ flags) {
+                                                    unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= c.n) return;
   const int id = __float_as_int(reinterpret_cast<const float*>(pos + p)[3]);
-  if (id < 0 || id >= c.n) return;  // ids are 0..n-1
+  if (id < 0 || id >= c.n) {  // ids are 0..n-1; refuse rather than write outside
+    atomicOr(flags, kFlagRange);
+    return;
+  }
   kout[id] = key[p];
   vout[id] = p;
 }
@@ -1076,10 +1079,12 @@ __global__ __launch_bounds__(1024) void k_neighbors(
     });
   if (active && !dense) walk_cells<false>(s, p.x, p.y, p.z, rx, ry, rz, g, c, cv.start, pred, w);
   if (c.count_only) {
-    // statistics: add this wavefront's pairs to *nbr_total, store nothing
+    // statistics: add this wavefront's pairs to *nbr_total and keep each
+    // particle's count; no list is stored
     unsigned long long v = active ? static_cast<unsigned long long>(w.count) : 0ull;
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if ((threadIdx.x & (kWave - 1)) == 0 && v != 0ull) atomicAdd(nbr_total, v);
+    if (L.active) ncount[s] = active ? w.count : 0;
     return;
   }
   // The encoded length of one list stays below 2^31 words (at most two words
@@ -2278,6 +2283,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                     c.seg,       c.lane_rep, c.lane_info, runs ? c.run_end : nullptr,
                     PK(0)};
   c.lists_partial = !rec && k.coop && !k.full_lists;
+  c.last_runs = runs;
   auto fill = [&](float4* a, const float4* ids) {
     if (runs)
       k_run_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(k, c.run_no, c.rep_list, a, ids,

@@ -158,7 +158,9 @@ int fluid_hip_sync(fluid_hip_ctx* ctx);
  *   floor(p * (1/h)) (core/src/core.cpp:28-34).  Needs record_diag=1.
  * sorted_ids[n]: original ids in the solver's sorted order (scene, Z-order
  *   cell key, original id).  Needs record_diag=1 for the cell keys, not ids.
- * nbr_count[n]: neighbor count of each particle (core/src/core.cpp:205-247).
+ * nbr_count[n]: neighbor count of each particle (core/src/core.cpp:205-247);
+ *   when the last substep kept no list for some particles (dense cells, runs
+ *   of identical particles), a counting walk of its grid supplies them.
  * rho[n], lambda[n]: density and lambda of the last constraint iteration.
  * Any pointer may be NULL.
  */

@@ -100,7 +100,8 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
          dtype=np.float32, mutation=0, threads=None, params64=None, nbr_cap=None):
     """Run `nsteps` CPU steps.  pos, vel: (3, n) arrays.  Returns a dict with
     pos, vel, time and, with diag=True, the last step's grid/neighbor/rho data
-    (original particle order).
+    (original particle order); diag="light" returns only the last step's
+    neighbor counts, pair total, rho and lambda.
 
     which: "oracle" (this project's restatement) or "ref" (the reference
     core.cpp).  dtype float64 runs the FP64 oracle; `params64` then supplies
@@ -120,7 +121,16 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
     t = np.array([time], dtype=dtype)
     out = {}
     d = None
-    if diag:
+    light = diag == "light"
+    if light:
+        # per-particle sums only (neighbor counts, rho, lambda): no lists, so
+        # piled-up states with 1e9 pairs stay cheap
+        out["nbr_count"] = np.zeros(n, np.int32)
+        out["rho"] = np.zeros(n, dtype)
+        out["lambda"] = np.zeros(n, dtype)
+        d = OracleDiag(None, None, out["nbr_count"].ctypes.data_as(C.POINTER(C.c_int)), None, 0,
+                       0, out["rho"].ctypes.data, out["lambda"].ctypes.data, None)
+    elif diag:
         cap = int(nbr_cap) if nbr_cap else max(1, n * 64)
         out["cell"] = np.zeros(3 * n, np.int32)
         out["sorted_ids"] = np.zeros(n, np.int32)
@@ -155,13 +165,15 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
             C.byref(d) if d is not None else None)
     if rc != 0:
         raise RuntimeError(f"{which} step failed rc={rc}")
-    if d is not None and d.nbr_total > d.nbr_cap:
+    if d is not None and not light and d.nbr_total > d.nbr_cap:
         # dense scenes: rerun with a list buffer of the exact size
         return step(params, pos_in, vel_in, nsteps=nsteps, time=time, diag=True, which=which,
                     dtype=dtype, mutation=mutation, threads=threads, params64=params64,
                     nbr_cap=int(d.nbr_total))
     out["pos"], out["vel"], out["time"] = pos, vel, t[0]
-    if diag:
+    if light:
+        out["nbr_total"] = int(d.nbr_total)
+    elif diag:
         out["cell"] = out["cell"].reshape(n, 3)
         out["nbr_ids"] = out["nbr_ids"][: d.nbr_total]
         out["pred0"] = out["pred0"].reshape(3, n)

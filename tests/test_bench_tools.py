@@ -54,8 +54,13 @@ def test_traffic_summary_scales_and_orders_dispatches(tmp_path):
             write += [(did, "k_lambda", "WRITE_SIZE", v / 2)]
     _pmc_db(str(tmp_path / "pmc_fetch" / "f_results.db"), fetch)
     _pmc_db(str(tmp_path / "pmc_write" / "w_results.db"), write)
+    # L2 hits and misses of the same dispatches: 3 hits per miss in the window
+    l2 = []
+    for did, name, _, v in fetch:
+        l2 += [(did, name, "TCC_HIT_sum", 3.0 * v), (did, name, "TCC_MISS_sum", v)]
+    _pmc_db(str(tmp_path / "pmc_l2" / "l_results.db"), l2)
     out = str(tmp_path / "traffic.json")
-    text = bench.summarize_traffic(str(tmp_path), 3, out)
+    text = bench.summarize_traffic(str(tmp_path), out, steps=3)
     d = json.load(open(out))
     assert abs(d["copy_calibration"]["fetch_scale"] - 2.0) < 1e-12
     for k in ("lambda", "delta"):
@@ -64,7 +69,25 @@ def test_traffic_summary_scales_and_orders_dispatches(tmp_path):
         # only the last three substeps (10 KiB each) count, both template variants
         assert abs(r["fetch_bytes_per_launch"] - 10.0 * 1024 * 2.0) < 1e-6
         assert abs(r["write_bytes_per_launch"] - 5.0 * 1024) < 1e-6
+        assert abs(r["l2_hit_rate"] - 0.75) < 1e-12
     assert "lambda" in text and "delta" in text
+
+
+def test_traffic_is_reported_only_for_the_recorded_window_and_config(tmp_path, monkeypatch):
+    prof = tmp_path / "profiles"
+    prof.mkdir()
+    cfg = {"block": 256, "xcd_remap": False, "wave_mode": 2, "brick_edge": 1, "graph": False}
+    rec = {"run": {"dims": [128, 128, 64], "warmup": 5, "steps": 20, "config": cfg},
+           "kernels": {"lambda": {"hbm_bytes_per_launch": 123.0, "l2_hit_rate": 0.9}}}
+    (prof / "traffic_million_block_warmup5_steps20.json").write_text(json.dumps(rec))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    run = {"dims": (128, 128, 64), "warmup": 5, "steps": 20, "config": dict(cfg)}
+    t, hit, src = bench.measured_traffic("lambda", run)
+    assert t == 123.0 and hit == 0.9 and src.endswith("steps20.json")
+    # another brick edge, another window: no traffic, and the reason says why
+    for other in (dict(run, config=dict(cfg, brick_edge=4)), dict(run, warmup=30, steps=120)):
+        t, hit, why = bench.measured_traffic("lambda", other)
+        assert t is None and hit is None and "no committed counter summary" in why
 
 
 def test_rocprof_window_skips_instant_dispatches(tmp_path):
