@@ -627,9 +627,9 @@ def main():
     s.sync()
     if prof:
         s.profile(True)
+    grows_before = s.stats()["grow_events"]
     sync_all()
     t0 = time.perf_counter()
-    grows_before = s.stats()["grow_events"]
     s.step(args.steps, graph=args.graph)  # one sync at the end; grows tables if needed
     sync_all()
     elapsed = time.perf_counter() - t0

@@ -45,8 +45,38 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+APPDIR = os.path.join(HERE, "app")
+BINDIR = os.path.join(HERE, "bin")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+# host tools over the C ABI (plain C++, no HIP headers): the app with the
+# reference's command line, the parity harness's diff tool and the
+# inverse-problem driver
+TOOLS = ["fluidsim_hip", "fluid_framediff", "fluid_fit"]
+
+
+def build_tools(verbose: bool = False) -> list:
+    """Compile the host tools in app/ against the in-tree library."""
+    os.makedirs(BINDIR, exist_ok=True)
+    out = []
+    for tool in TOOLS:
+        src = os.path.join(APPDIR, tool + ".cpp")
+        if not os.path.exists(src):
+            continue
+        exe = os.path.join(BINDIR, tool)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
+               "-Wno-unused-parameter", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{APPDIR}", src, "-o", exe + ".tmp",
+               f"-L{LIBDIR}", "-lfluid_hip", "-pthread", "-Wl,-rpath,$ORIGIN/../lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source for gfx950 and link libfluid_hip.so in-tree."""
+    """Compile every source for gfx950 and link libfluid_hip.so in-tree, then
+    the host tools."""
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
     for src in SOURCES:
@@ -67,6 +97,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_tools(verbose)
     return LIB
 
 

@@ -106,10 +106,10 @@ __device__ __forceinline__ float3 project_bwd(float3 v, unsigned mask, const Sce
 // Transpose of restitution/friction's Jacobian (each contact plane's map is
 // symmetric), planes in reverse order.
 __device__ __forceinline__ float3 restitute_bwd(float3 v, unsigned mask, const ScenePlanes& sp) {
-  const float t = 1.0f - sp.friction;
   for (int p = sp.n_planes - 1; p >= 0; --p) {
     if (!((mask >> p) & 1u)) continue;
-    const float a = ((mask >> (8 + p)) & 1u) ? -sp.restitution : 1.0f;
+    const float t = 1.0f - sp.fric_of[p];
+    const float a = ((mask >> (8 + p)) & 1u) ? -sp.rest_of[p] : 1.0f;
     const float4 pl = sp.planes[p];
     const float vn = pl.x * v.x + pl.y * v.y + pl.z * v.z;
     const float k = (a - t) * vn;

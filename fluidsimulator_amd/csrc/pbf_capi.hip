@@ -64,10 +64,31 @@ bool finite_all(const fluid_hip_params& p) {
                      p.vort_epsilon, p.vort_norm_eps, p.gravity[0], p.gravity[1], p.gravity[2]};
   for (float x : v)
     if (!std::isfinite(x)) return false;
-  for (int i = 0; i < p.n_planes && i < FLUID_HIP_MAX_PLANES; ++i)
+  for (int i = 0; i < p.n_planes && i < FLUID_HIP_MAX_PLANES; ++i) {
     for (int k = 0; k < 4; ++k)
       if (!std::isfinite(p.planes[i][k])) return false;
+    if (p.plane_props &&
+        !(std::isfinite(p.plane_restitution_of[i]) && std::isfinite(p.plane_friction_of[i])))
+      return false;
+  }
   return true;
+}
+
+// Restitution e and friction f of plane i: its own with plane_props, else
+// the scene's (the reference applies one pair to every plane).
+float plane_e(const fluid_hip_params& p, int i) {
+  return p.plane_props ? p.plane_restitution_of[i] : p.plane_restitution;
+}
+float plane_f(const fluid_hip_params& p, int i) {
+  return p.plane_props ? p.plane_friction_of[i] : p.plane_friction;
+}
+
+// The contact pass of core.cpp:573-612 runs (e > 0 || f > 0, for some plane).
+bool contact_pass_on(const fluid_hip_params& p) {
+  if (!p.plane_props) return p.plane_restitution > 0.0f || p.plane_friction > 0.0f;
+  for (int i = 0; i < p.n_planes; ++i)
+    if (plane_e(p, i) > 0.0f || plane_f(p, i) > 0.0f) return true;
+  return false;
 }
 
 // Fields every scene of a batch must share.
@@ -104,10 +125,10 @@ int validate(int n_scenes, const fluid_hip_params* params) {
   return FLUID_HIP_OK;
 }
 
-void compute_consts(Ctx& c) {
-  // The loop invariants of core/src/core.cpp:138-148 and :270-274, in the
-  // same float expressions.
-  const fluid_hip_params& p = c.params[0];
+// The loop invariants of core/src/core.cpp:138-148 and :270-274, in the
+// same float expressions, for the scenes of `params` (n_scenes entries).
+void compute_consts_of(Ctx& c, const fluid_hip_params* params, int n_scenes) {
+  const fluid_hip_params& p = params[0];
   fluidhip::StepConsts k{};
   k.n = c.n;
   k.n_scenes = c.n_scenes;
@@ -167,27 +188,34 @@ void compute_consts(Ctx& c) {
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423
   c.vort = p.enable_vorticity && p.vort_epsilon != 0.0f;  // core.cpp:468
   c.restitution_any = false;
-  for (const auto& q : c.params)
-    if ((q.plane_restitution > 0.0f || q.plane_friction > 0.0f) && q.n_planes > 0)
-      c.restitution_any = true;
+  for (int s = 0; s < n_scenes; ++s) {
+    const fluid_hip_params& q = params[s];
+    if (contact_pass_on(q) && q.n_planes > 0) c.restitution_any = true;
+  }
+}
+
+void compute_consts(Ctx& c) { compute_consts_of(c, c.params.data(), c.n_scenes); }
+
+// Gravity, planes, restitution and friction of one scene, as the kernels read them.
+fluidhip::ScenePlanes scene_table(const fluid_hip_params& p) {
+  fluidhip::ScenePlanes q;
+  std::memset(&q, 0, sizeof(q));
+  q.gx = p.gravity[0];
+  q.gy = p.gravity[1];
+  q.gz = p.gravity[2];
+  q.n_planes = p.n_planes;
+  q.restitution_on = contact_pass_on(p) ? 1 : 0;
+  for (int i = 0; i < p.n_planes; ++i) {
+    q.planes[i] = make_float4(p.planes[i][0], p.planes[i][1], p.planes[i][2], p.planes[i][3]);
+    q.rest_of[i] = plane_e(p, i);
+    q.fric_of[i] = plane_f(p, i);
+  }
+  return q;
 }
 
 std::vector<fluidhip::ScenePlanes> scene_tables(const Ctx& c) {
   std::vector<fluidhip::ScenePlanes> t(c.n_scenes);
-  for (int s = 0; s < c.n_scenes; ++s) {
-    const fluid_hip_params& p = c.params[s];
-    fluidhip::ScenePlanes& q = t[s];
-    std::memset(&q, 0, sizeof(q));
-    q.gx = p.gravity[0];
-    q.gy = p.gravity[1];
-    q.gz = p.gravity[2];
-    q.n_planes = p.n_planes;
-    q.restitution = p.plane_restitution;
-    q.friction = p.plane_friction;
-    q.restitution_on = (p.plane_restitution > 0.0f || p.plane_friction > 0.0f) ? 1 : 0;
-    for (int i = 0; i < p.n_planes; ++i)
-      q.planes[i] = make_float4(p.planes[i][0], p.planes[i][1], p.planes[i][2], p.planes[i][3]);
-  }
+  for (int s = 0; s < c.n_scenes; ++s) t[s] = scene_table(c.params[s]);
   return t;
 }
 
@@ -246,6 +274,8 @@ void free_all(Ctx& c) {
     hipEventDestroy(q.b);
   }
   for (auto e : c.prof.pool) hipEventDestroy(e);
+  if (c.dl_event) hipEventDestroy(c.dl_event);
+  c.dl_event = nullptr;
   c.prof.pending.clear();
   c.prof.pool.clear();
 }
@@ -787,6 +817,51 @@ int fluid_hip_download(fluid_hip_ctx* ctx, float* pos, float* vel, float* time) 
   }
   HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
   if (time) *time = c.time;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_download_async(fluid_hip_ctx* ctx, float* pos, float* vel) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.uploaded) return fail(FLUID_HIP_ESTATE, "nothing uploaded");
+  if (c.n > 0 && (!pos || !vel)) return fail(FLUID_HIP_EINVAL, "pos/vel is null");
+  if (!c.dl_event) HIP_TRY(hipEventCreateWithFlags(&c.dl_event, hipEventDisableTiming), "hipEventCreate");
+  const size_t bytes = sizeof(float) * 3 * static_cast<size_t>(c.n);
+  if (c.n > 0) {
+    float* dp = c.stage;
+    float* dv = c.stage + 3 * static_cast<size_t>(c.n);
+    HIP_TRY(fluidhip::enqueue_download(c, dp, dv), "download kernel");
+    HIP_TRY(hipMemcpyAsync(pos, dp, bytes, hipMemcpyDeviceToHost, c.stream), "download pos");
+    HIP_TRY(hipMemcpyAsync(vel, dv, bytes, hipMemcpyDeviceToHost, c.stream), "download vel");
+  }
+  HIP_TRY(hipEventRecord(c.dl_event, c.stream), "hipEventRecord");
+  c.dl_time = c.time;
+  c.dl_pending = true;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_download_wait(fluid_hip_ctx* ctx, float* time) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  Ctx& c = ctx->c;
+  if (!c.dl_pending) return fail(FLUID_HIP_ESTATE, "no asynchronous download queued");
+  HIP_TRY(hipEventSynchronize(c.dl_event), "hipEventSynchronize");
+  c.dl_pending = false;
+  if (time) *time = c.dl_time;
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_host_alloc(void** out, int64_t bytes) {
+  if (!out || bytes < 0) return fail(FLUID_HIP_EINVAL, "bad arguments");
+  *out = nullptr;
+  HIP_TRY(hipHostMalloc(out, static_cast<size_t>(bytes > 0 ? bytes : 1), hipHostMallocDefault),
+          "hipHostMalloc");
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr), "hipHostFree");
   return FLUID_HIP_OK;
 }
 

@@ -130,6 +130,9 @@ struct Ctx {
 
   // staging for transfers
   float* stage = nullptr;  // 6 * n
+  hipEvent_t dl_event = nullptr;  // end of the last asynchronous download
+  float dl_time = 0.0f;           // time of the state it downloads
+  bool dl_pending = false;
 
   // graph of one substep
   hipGraph_t graph = nullptr;

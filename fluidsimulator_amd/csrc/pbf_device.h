@@ -62,11 +62,12 @@ struct StepConsts {
 struct ScenePlanes {
   float gx, gy, gz;
   int n_planes;
-  float restitution;   // plane_restitution
-  float friction;      // plane_friction
-  int restitution_on;  // e > 0 || f > 0 (core.cpp:573)
-  int pad;
+  int restitution_on;  // e > 0 || f > 0 for some plane (core.cpp:573)
+  int pad[3];
   float4 planes[kMaxPlanes];  // (nx, ny, nz, d)
+  float rest_of[kMaxPlanes];  // restitution e of each plane (the scene's
+                              // plane_restitution unless given per plane)
+  float fric_of[kMaxPlanes];  // friction f of each plane
 };
 
 // Per-scene grid descriptor, rebuilt every substep from the predicted
@@ -105,6 +106,34 @@ __device__ __forceinline__ float pow_ratio(float ratio, int n) {
   }
   return powf(ratio, static_cast<float>(n));
 }
+
+// Spiky factor of a pair at squared distance r2 < h2: pairs closer than
+// min_r share one precomputed factor (core.cpp:303-304 clamps r there), so
+// only the others pay for the IEEE square root.
+__device__ __forceinline__ float pair_spiky(float r2, const StepConsts& c) {
+  if (r2 < c.min_r2) return c.f_min;
+  return spiky(sqrtf(r2), c);
+}
+
+// The neighbor predicate: candidate q is a neighbor of a particle at p when
+// |p - q|^2 < h^2, both at their predicted positions p^0 (core.cpp:140-142
+// and 222-245).  Lists and neighbor masks are built with it, and cooperative
+// walks retest it instead of reading stored lists, so all must use this one
+// expression.
+__device__ __forceinline__ bool near0(const float3& p, const float3& q, const StepConsts& c) {
+  const float ex = p.x - q.x;
+  const float ey = p.y - q.y;
+  const float ez = p.z - q.z;
+  return ex * ex + ey * ey + ez * ez < c.h2;
+}
+
+// Lanes whose own cell holds at least this many particles are served by
+// wavefront-cooperative walks (one per distinct dense cell in the
+// wavefront); the others walk alone.
+constexpr int kDenseCell = 32;
+// ... and so are lanes with at least this many neighbors (or, while the
+// lists are built, candidates).
+constexpr int kHeavyList = 256;
 
 // Z-order of a cell inside its brick: the low `bits` bits of x, y, z
 // interleaved as x0 y0 z0 x1 y1 z1 ...
@@ -201,7 +230,8 @@ __device__ __forceinline__ unsigned project_planes(float& qx, float& qy, float& 
   return mask;
 }
 
-// Plane restitution and friction (core.cpp:573-612).  Returns the planes in
+// Plane restitution and friction (core.cpp:573-612), with each plane's own
+// e and f (equal to the scene's globals unless set per plane).  Returns the planes in
 // contact (bit p) and, among those, the ones the velocity approached
 // (bit 8 + p): with them the map is linear, v -> (1-f) v + (a - (1-f)) (n.v) n
 // with a = -e when approaching and 1 otherwise.
@@ -217,13 +247,13 @@ __device__ __forceinline__ unsigned restitute(float& vx, float& vy, float& vz, f
       float vn_new = vn;
       mask |= 1u << p;
       if (vn < 0.0f) {
-        vn_new = -sp.restitution * vn;
+        vn_new = -sp.rest_of[p] * vn;
         mask |= 1u << (8 + p);
       }
       const float tx = vx - vn * pl.x;
       const float ty = vy - vn * pl.y;
       const float tz = vz - vn * pl.z;
-      const float scale = 1.0f - sp.friction;
+      const float scale = 1.0f - sp.fric_of[p];
       vx = tx * scale + vn_new * pl.x;
       vy = ty * scale + vn_new * pl.y;
       vz = tz * scale + vn_new * pl.z;

@@ -3,21 +3,24 @@
 // Functional spec: the reference CPU step `fluid::step`
 // (core/src/core.cpp:119-615).  Everything here is written for CDNA4:
 // 64-lane wavefronts, float4 AoS state resident in HBM, a dense per-scene
-// cell table of 4x4x4-cell bricks (Z-order inside a brick), an atomic
-// counting sort followed by a by-id rank inside each cell, and CSR neighbor
-// lists allocated per workgroup.
+// cell table of e x e x e-cell bricks (Z-order inside a brick), a stable LSD
+// radix sort of the cell keys into the reference's (cell, id) order, and
+// run-length-encoded neighbor lists allocated per workgroup.
 //
-// Phase order of one substep (one launch each; every launch covers all
-// scenes of a batch):
+// Phase order of one substep (one launch each unless noted; every launch
+// covers all scenes of a batch):
 //   predict_bbox  v' = v + g dt, pred0 = x + v' dt, cell = floor(pred0/h),
-//                 per-scene bounding box of cells          (core.cpp:150-161)
+//                 per-tile bounding boxes of cells        (core.cpp:150-161)
 //   grid_setup    per-scene table origin/extent, key offsets, counters
-//   cell_count    key of each particle, atomic cell histogram
+//   cell_count    key of each particle, wavefront-aggregated cell histogram
 //   scan x3       exclusive scan of the histogram -> cell starts
-//   scatter       slot = start[key] + rank
-//   reorder       rank by original id inside each cell (the reference's
-//                 (cell, id) order, core.cpp:173-183), gather state into
-//                 sorted order and recompute pred0 there
+//   sort          keys in original-id order, then 8-bit LSD radix passes
+//                 (digit histogram, its scan, stable ballot-ranked scatter):
+//                 sorting stably by key leaves each cell's particles in
+//                 ascending id, the reference's order (core.cpp:164-203)
+//   reorder       gather each particle's state into its slot, recompute pred0
+//   runs          runs of identical consecutive particles and the lane map of
+//                 the gather passes (dense cells from a wave boundary)
 //   neighbors     27 cells in (dz, dy, dx) order, each by ascending id,
 //                 j != i and r^2 < h^2 on pred0       (core.cpp:205-247)
 //   lambda/delta  x solver_iterations, Jacobi double buffer (core.cpp:277-408);
@@ -358,13 +361,6 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
   }
 }
 
-// Lanes whose own cell holds at least this many particles are served by
-// wavefront-cooperative walks (one per distinct dense cell in the
-// wavefront); the others walk alone.
-constexpr int kDenseCell = 32;
-// ... and so are lanes with at least this many neighbors (or, while the
-// lists are built, candidates).
-constexpr int kHeavyList = 256;
 
 // Distinct predicted positions p^0 among the lanes of a walk up to which a
 // tile's neighbor sets are decided with one ballot per position, and
@@ -673,16 +669,6 @@ struct RunWriter {
   }
 };
 
-// The neighbor predicate: candidate q is a neighbor of a particle at p when
-// |p - q|^2 < h^2, both at their predicted positions p^0 (core.cpp:140-142
-// and 222-245).  Lists are built with it, and cooperative walks retest it
-// instead of reading stored lists, so both must use this one expression.
-__device__ __forceinline__ bool near0(const float3& p, const float3& q, const StepConsts& c) {
-  const float ex = p.x - q.x;
-  const float ey = p.y - q.y;
-  const float ez = p.z - q.z;
-  return ex * ex + ey * ey + ez * ez < c.h2;
-}
 
 // Candidates of a particle: the 27 cells around its own in (dz, dy, dx)
 // order, each cell's slots in ascending order, i.e. ascending original id.
@@ -1363,13 +1349,6 @@ __device__ __forceinline__ void gather(const StepConsts& c, const NbrView& nv, c
     for_each_neighbor(nv, L.s, load, [&](int, const auto& q) { ops.pair(q); });
 }
 
-// Spiky factor of a pair at squared distance r2 < h2: pairs closer than
-// min_r share one precomputed factor (core.cpp:303-304 clamps r there), so
-// only the others pay for the IEEE square root.
-__device__ __forceinline__ float pair_spiky(float r2, const StepConsts& c) {
-  if (r2 < c.min_r2) return c.f_min;
-  return spiky(sqrtf(r2), c);
-}
 
 // Candidate records loaded once per tile and broadcast lane to lane.
 struct EPos {

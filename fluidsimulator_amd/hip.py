@@ -36,6 +36,8 @@ class FluidHipParams(C.Structure):
         ("vort_epsilon", C.c_float), ("vort_norm_eps", C.c_float),
         ("gravity", C.c_float * 3), ("n_planes", C.c_int32),
         ("planes", (C.c_float * 4) * MAX_PLANES),
+        ("plane_props", C.c_int32), ("plane_restitution_of", C.c_float * MAX_PLANES),
+        ("plane_friction_of", C.c_float * MAX_PLANES),
     ]
 
 
@@ -76,6 +78,10 @@ _SIGS = {
                                           C.POINTER(C.c_int64)]),
     "fluid_hip_get_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "fluid_hip_get_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
+    "fluid_hip_download_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fluid_hip_download_wait": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "fluid_hip_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
+    "fluid_hip_host_free": (C.c_int, [C.c_void_p]),
     "fluid_hip_profile": (C.c_int, [C.c_void_p, C.c_int32]),
     "fluid_hip_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double),
                                          C.POINTER(C.c_int64)]),
@@ -89,6 +95,7 @@ _OPTIONAL_SIGS: dict = {
     "fluid_hip_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
 }
+
 
 
 def exported_symbols():
@@ -156,6 +163,14 @@ def pack_params(p: Params) -> FluidHipParams:
     for i, pl in enumerate(p.planes):
         for k in range(4):
             q.planes[i][k] = float(np.float32(pl[k]))
+    props = getattr(p, "plane_props", None)
+    if props:
+        if len(props) != len(p.planes):
+            raise ValueError("plane_props needs one (restitution, friction) pair per plane")
+        q.plane_props = 1
+        for i, (e, f) in enumerate(props):
+            q.plane_restitution_of[i] = float(np.float32(e))
+            q.plane_friction_of[i] = float(np.float32(f))
     return q
 
 

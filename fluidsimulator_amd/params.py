@@ -39,6 +39,9 @@ class Params:
     vort_norm_eps: float = 1e-6
     external_forces: tuple = (0.0, -9.8, 0.0)
     planes: list = field(default_factory=list)  # [(nx, ny, nz, d), ...]
+    # Extension beyond fluid::Params: one (restitution, friction) pair per
+    # plane, or None for plane_restitution / plane_friction on every plane.
+    plane_props: list | None = None
 
     def f32(self, name: str) -> float:
         """The field as the float32 value the C++ struct would hold."""
@@ -48,6 +51,8 @@ class Params:
         p = dataclasses.replace(self, **kw)
         if "planes" not in kw:
             p.planes = list(self.planes)
+        if "plane_props" not in kw and self.plane_props is not None:
+            p.plane_props = list(self.plane_props)
         return p
 
     def add_plane_normalized(self, nx, ny, nz, d):

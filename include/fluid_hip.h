@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FLUID_HIP_ABI_VERSION 1
+#define FLUID_HIP_ABI_VERSION 2
 #define FLUID_HIP_MAX_PLANES 8
 
 /* Status codes. */
@@ -71,6 +71,15 @@ typedef struct fluid_hip_params {
   float gravity[3];          /* Params::external_forces */
   int32_t n_planes;          /* <= FLUID_HIP_MAX_PLANES */
   float planes[FLUID_HIP_MAX_PLANES][4]; /* (nx, ny, nz, d): n.p - d >= 0 inside */
+  /* Extension with no counterpart in the reference, whose restitution and
+   * friction hold for every plane (core/src/core.cpp:573-612): with
+   * plane_props = 1, plane p bounces with plane_restitution_of[p] and damps
+   * with plane_friction_of[p] (the scene's contact pass runs when any plane
+   * has either above zero); with 0 the two globals above hold for every
+   * plane, which is the reference's behaviour. */
+  int32_t plane_props;
+  float plane_restitution_of[FLUID_HIP_MAX_PLANES];
+  float plane_friction_of[FLUID_HIP_MAX_PLANES];
 } fluid_hip_params;
 
 /* Capacity knobs for fluid_hip_create; zero fields take defaults. */
@@ -134,6 +143,21 @@ int fluid_hip_set_stream(fluid_hip_ctx* ctx, void* stream);
 /* pos, vel: host SoA blocks of 3*n floats, original particle order. */
 int fluid_hip_upload(fluid_hip_ctx* ctx, const float* pos, const float* vel, float time);
 int fluid_hip_download(fluid_hip_ctx* ctx, float* pos, float* vel, float* time);
+
+/* Asynchronous download for output while the device keeps stepping (the
+ * reference's app downloads nothing: its state lives on the host).  Queues
+ * the committed state's conversion to original order and its copy into
+ * pos/vel (3*n floats each; pinned memory from fluid_hip_host_alloc makes
+ * the copy asynchronous) on the context's stream and returns at once;
+ * fluid_hip_download_wait() blocks until that copy has landed and returns
+ * the time of the downloaded state.  Steps queued in between run after the
+ * copy, in stream order. */
+int fluid_hip_download_async(fluid_hip_ctx* ctx, float* pos, float* vel);
+int fluid_hip_download_wait(fluid_hip_ctx* ctx, float* time);
+
+/* Page-locked host memory (for fluid_hip_download_async), and its release. */
+int fluid_hip_host_alloc(void** out, int64_t bytes);
+int fluid_hip_host_free(void* ptr);
 
 /* ---- stepping ------------------------------------------------------------ */
 

@@ -33,7 +33,7 @@ class OracleParams(C.Structure):
         ("plane_restitution", C.c_double), ("plane_friction", C.c_double),
         ("vort_epsilon", C.c_double), ("vort_norm_eps", C.c_double),
         ("gravity", C.c_double * 3), ("n_planes", C.c_int), ("mutation", C.c_int),
-        ("planes", C.POINTER(C.c_double)),
+        ("planes", C.POINTER(C.c_double)), ("plane_props", C.POINTER(C.c_double)),
     ]
 
 
@@ -93,7 +93,15 @@ def _pack_params(p, mutation=0):
     op.n_planes = planes.shape[0]
     op.mutation = int(mutation)
     op.planes = planes.ctypes.data_as(C.POINTER(C.c_double)) if planes.size else None
-    return op, planes
+    props = getattr(p, "plane_props", None)
+    keep = [planes]
+    if props:
+        pp = np.ascontiguousarray(np.asarray(props, np.float32).reshape(-1, 2).astype(np.float64))
+        if pp.shape[0] != planes.shape[0]:
+            raise ValueError("plane_props needs one (restitution, friction) pair per plane")
+        op.plane_props = pp.ctypes.data_as(C.POINTER(C.c_double))
+        keep.append(pp)
+    return op, keep
 
 
 def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",

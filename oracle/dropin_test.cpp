@@ -5,11 +5,15 @@
 // (core/src/init.cpp:72-117) and stepped once with the reference's
 // fluid::step (core/src/core.cpp:119-615) and once with fluid::hip_step /
 // fluid::HipSession (include/fluid_hip_backend.hpp).  Exit status 0 when
-// every position, velocity and the time agree bitwise.
+// every position, velocity and the time agree bitwise.  With arguments
+// SCENE STEPS [FLAGS] the scene comes from a scene file through the
+// reference's own init_scene_from_json instead.
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "fluid/core.h"
 #include "fluid/init.h"
@@ -56,7 +60,37 @@ void scene(fluid::Params& p, fluid::State& s, bool flags) {
 
 }  // namespace
 
-int main() {
+// A scene file through the reference's own loader (init_scene_from_json,
+// core/src/init.cpp:158-418) with the app's settings for it, then `steps`
+// substeps with fluid::step and with fluid::HipSession.
+int run_scene_file(const char* path, int steps, bool flags) {
+  fluid::Params p;
+  fluid::State ref = fluid::make_state(0);
+  std::string error;
+  if (!fluid::init_scene_from_json(path, p, ref, &error)) {
+    std::printf("scene %s failed to load: %s\n", path, error.c_str());
+    return 1;
+  }
+  p.dt = static_cast<float>(1.0 / 120);  // --steps-per-sec 120
+  p.plane_restitution = 0.05f;           // --plane-restitution 0.05
+  p.plane_friction = 0.1f;               // --plane-friction 0.1
+  if (flags) {                           // --enable-scorr --enable-xsph --enable-vorticity
+    p.enable_scorr = true;
+    p.enable_xsph = true;
+    p.enable_vorticity = true;
+  }
+  fluid::State one = ref;
+  fluid::HipSession session(p, one);
+  for (int k = 0; k < steps; ++k) fluid::step(p, ref);
+  session.step(steps);
+  session.download(one);
+  const bool ok = same_state(ref, one);
+  std::printf("scene=%s flags=%d particles=%zu session(%d steps) bitwise=%d\n", path, flags ? 1 : 0,
+              ref.size(), steps, ok ? 1 : 0);
+  return ok ? 0 : 1;
+}
+
+int main(int argc, char** argv) {
   int ndev = 0;
   const char* err = nullptr;
   if (!fluid::hip_device_available(&ndev, &err)) {
@@ -64,6 +98,8 @@ int main() {
     return 2;
   }
   std::printf("hip_version=%d devices=%d\n", fluid::hip_version(), ndev);
+  if (argc >= 3)  // dropin_test SCENE.json STEPS [flags]
+    return run_scene_file(argv[1], std::atoi(argv[2]), argc >= 4 && std::atoi(argv[3]) != 0);
   int failures = 0;
   for (int flags = 0; flags < 2; ++flags) {
     fluid::Params p;

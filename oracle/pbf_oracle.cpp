@@ -144,6 +144,10 @@ struct OracleParams {
   int n_planes;
   int mutation;
   const double* planes;  // n_planes x (nx, ny, nz, d)
+  // Extension beyond the reference (whose restitution and friction hold for
+  // every plane): n_planes x (restitution, friction), or null for the two
+  // globals above on every plane.
+  const double* plane_props;
 };
 
 // Optional per-step outputs of the last step, in original particle order.
@@ -548,10 +552,20 @@ void oracle_step_impl(const OracleParams& P, size_t n, T* px_, T* py_, T* pz_,
     }
   }
 
-  // plane restitution / friction — core.cpp:573-612
+  // plane restitution / friction — core.cpp:573-612 (with plane_props each
+  // plane's own pair; the pass runs when any plane has one above zero)
   const T rest = static_cast<T>(P.plane_restitution);
   const T fric = static_cast<T>(P.plane_friction);
-  if ((rest > T(0) || fric > T(0)) && P.n_planes > 0) {
+  auto rest_of = [&](int p) { return P.plane_props ? static_cast<T>(P.plane_props[2 * p]) : rest; };
+  auto fric_of = [&](int p) {
+    return P.plane_props ? static_cast<T>(P.plane_props[2 * p + 1]) : fric;
+  };
+  bool contact = rest > T(0) || fric > T(0);
+  if (P.plane_props) {
+    contact = false;
+    for (int p = 0; p < P.n_planes; ++p) contact = contact || rest_of(p) > T(0) || fric_of(p) > T(0);
+  }
+  if (contact && P.n_planes > 0) {
 #pragma omp parallel for
     for (long i = 0; i < N; ++i) {
       T vx = vx_[i], vy = vy_[i], vz = vz_[i];
@@ -562,11 +576,11 @@ void oracle_step_impl(const OracleParams& P, size_t n, T* px_, T* py_, T* pz_,
         if (sd <= T(0)) {
           const T vn = nx * vx + ny * vy + nz * vz;
           T vn_new = vn;
-          if (vn < T(0)) vn_new = -rest * vn;
+          if (vn < T(0)) vn_new = -rest_of(p) * vn;
           const T tx = vx - vn * nx;
           const T ty = vy - vn * ny;
           const T tz = vz - vn * nz;
-          const T scale = T(1) - fric;
+          const T scale = T(1) - fric_of(p);
           vx = tx * scale + vn_new * nx;
           vy = ty * scale + vn_new * ny;
           vz = tz * scale + vn_new * nz;

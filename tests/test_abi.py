@@ -27,7 +27,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 
 
 def test_version_and_device_query():
-    assert hip.hip_version() == 1
+    assert hip.hip_version() == 2
     ok, count, err = hip.hip_device_available()
     assert ok == (count > 0)
     if not ok:

@@ -42,6 +42,11 @@ def _consts(p):
         dq=f32(p.scorr_dq_coeff), xsph=bool(p.enable_xsph) and f32(p.visc_c) != 0.0,
         c=f32(p.visc_c), e=f32(p.plane_restitution), f=f32(p.plane_friction),
         planes=np.asarray(p.planes, np.float32).reshape(-1, 4).astype(np.float64),
+        # (restitution, friction) of each plane: per plane when given, else the globals
+        props=(np.asarray(p.plane_props, np.float32).reshape(-1, 2).astype(np.float64)
+               if getattr(p, "plane_props", None) else
+               np.tile([f32(p.plane_restitution), f32(p.plane_friction)],
+                       (len(p.planes), 1)).astype(np.float64)),
     )
 
 
@@ -108,14 +113,14 @@ def substep(x, v, g, p, K=None):
         mu = torch.where(rho > 0, K["m"] / rho, torch.zeros_like(rho))
         dv = torch.zeros(n, 3, dtype=x.dtype).index_add(0, i, (W * mu[j])[:, None] * (vc[j] - vc[i]))
         vx = vc + K["c"] * dv
-    if (K["e"] > 0.0 or K["f"] > 0.0) and len(planes):
-        for pl in planes:
+    if bool((K["props"] > 0.0).any()) and len(planes):
+        for pl, (e, f) in zip(planes, K["props"]):
             nrm, dd = pl[:3], pl[3]
             contact = ((x1 @ nrm - dd) <= 0).detach()
             vn = vx @ nrm
-            vn_new = torch.where(vn < 0, -K["e"] * vn, vn)
+            vn_new = torch.where(vn < 0, -float(e) * vn, vn)
             vt = vx - vn[:, None] * nrm
-            out = vt * (1.0 - K["f"]) + vn_new[:, None] * nrm
+            out = vt * (1.0 - float(f)) + vn_new[:, None] * nrm
             vx = torch.where(contact[:, None], out, vx)
     return x1, vx
 
