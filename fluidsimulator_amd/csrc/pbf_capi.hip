@@ -12,7 +12,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -554,6 +556,142 @@ long long tape_bytes(const Ctx& c) {
     total += n * (16 * 3 + 32 * I + 4 * I + 4 + 16) + 4 * t.nbr_cap;
   total += static_cast<long long>(c.adj->ck_pos.size()) * 32 * n;
   return total;
+}
+
+// ---------------------------------------------------------------- no allocation
+//
+// fluid_hip_step_noalloc keeps every buffer of a substep in one file-scope
+// device array per GPU instead of memory of its own.  Host code never uses
+// the array's name as an address (on the host it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per
+// device, and only that address reaches kernels.
+
+constexpr size_t kNoAllocBytes = static_cast<size_t>(FLUID_HIP_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// The device address of this GPU's scratch array (the current device).
+int noalloc_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(FLUID_HIP_EINVAL, "device ordinal too large");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(FLUID_HIP_EHIP, "hipGetSymbolAddress(scratch) returned null");
+    g_noalloc_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return FLUID_HIP_OK;
+}
+
+// Lays the buffers of an n-particle, one-scene context out in [base, base +
+// bytes): the per-particle and fixed arrays first, then the rest split
+// between the neighbor lists (5/8) and the cell table (3/8).  Returns false
+// when they do not fit.
+bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
+  size_t off = 0;
+  bool ok = true;
+  auto take = [&](auto*& p, size_t count) {
+    using T = std::remove_reference_t<decltype(*p)>;
+    off = (off + 255) & ~static_cast<size_t>(255);
+    const size_t need = std::max<size_t>(count, 1) * sizeof(T);
+    if (off + need > bytes) {
+      ok = false;
+      p = nullptr;
+      return;
+    }
+    p = reinterpret_cast<T*>(base + off);
+    off += need;
+  };
+  const size_t n = static_cast<size_t>(c.n);
+  for (float4** q : {&c.posA, &c.velA, &c.posB, &c.velB, &c.pred[0], &c.pred[1], &c.pred[2]})
+    take(*q, n);
+  if (c.vort) take(c.omega, n);
+  take(c.rho, n);
+  take(c.cellc, n);
+  take(c.key, n);
+  for (int q = 0; q < 2; ++q) {
+    take(c.sort_k[q], n);
+    take(c.sort_v[q], n);
+  }
+  c.radix_blocks = static_cast<int>((n + fluidhip::kRadixTile - 1) / fluidhip::kRadixTile);
+  take(c.radix_hist, 256 * static_cast<size_t>(c.radix_blocks));
+  take(c.radix_off, 256 * static_cast<size_t>(c.radix_blocks) + 1);
+  take(c.radix_len, 1);
+  take(c.slot_cell, n);
+  take(c.slot_key, n);
+  take(c.noff, n);
+  take(c.nwords, n);
+  take(c.ncount, n);
+  take(c.bbox, 6);
+  take(c.tile_bbox, 6 * (n / 64 + 1));
+  take(c.run_no, n);
+  take(c.run_excl, n + 1);
+  take(c.rep_list, n);
+  take(c.run_end, n);
+  take(c.cell_no, n);
+  take(c.cell_excl, n + 1);
+  take(c.cell_first, n + 1);
+  take(c.lanes1, n + 1);
+  take(c.lanes2, n + 1);
+  take(c.off1, n + 1);
+  take(c.off2, n + 1);
+  c.lane_cap = 3ll * c.n + 64;
+  take(c.lane_rep, static_cast<size_t>(c.lane_cap));
+  take(c.lane_info, 3);
+  take(c.n_dev, 1);
+  take(c.grid, 1);
+  take(c.total_cells, 1);
+  take(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
+  take(c.nbr_total, 1);
+  take(c.need_cells, 1);
+  take(c.pair_total, 1);
+  take(c.seg, 2);
+  take(c.planes, 1);
+  take(c.flags, 1);
+  take(c.steps_started, 1);
+  if (!ok) return false;
+  off = (off + 255) & ~static_cast<size_t>(255);
+  const size_t rest = bytes - off;
+  const long long words = static_cast<long long>(rest / 8 * 5 / sizeof(int));
+  const long long cells = static_cast<long long>((rest - static_cast<size_t>(words) * sizeof(int)) /
+                                                 (2 * sizeof(int))) - 512;
+  if (words < 1 || cells < 1) return false;
+  take(c.nbr, static_cast<size_t>(words));
+  c.nbr_cap = words;
+  c.cell_cap = std::min<long long>(cells, INT_MAX - 1);
+  take(c.count, static_cast<size_t>(c.cell_cap));
+  take(c.start, static_cast<size_t>(c.cell_cap) + 1);
+  return ok;
+}
+
+// What must start at zero or at a known value: the scene tables, counters,
+// flags and bounding box, and the cell histogram (kernel stores, so the
+// sequence holds no memset node if a caller captures it).
+__global__ void k_noalloc_init(int n, int radix_len, fluidhip::ScenePlanes sp, int* seg,
+                               fluidhip::ScenePlanes* planes, int* n_dev, int* rlen, int* bbox,
+                               unsigned* flags, int* steps_started) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    seg[0] = 0;
+    seg[1] = n;
+    *planes = sp;
+    *n_dev = n;
+    *rlen = radix_len;
+    bbox[0] = bbox[1] = bbox[2] = INT_MAX;
+    bbox[3] = bbox[4] = bbox[5] = INT_MIN;
+    *flags = 0u;
+    *steps_started = 0;
+  }
+}
+
+__global__ void k_zero_ints(int* p, long long count) {
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += stride)
+    p[i] = 0;
 }
 
 }  // namespace
@@ -1410,6 +1548,72 @@ int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel,
   c.time = a.fin_time;
   if (dgravity)
     for (size_t i = 0; i < g.size(); ++i) dgravity[i] = static_cast<float>(g[i]);
+  return FLUID_HIP_OK;
+}
+
+int fluid_hip_step_noalloc(const fluid_hip_params* params, int64_t n, float* pos, float* vel,
+                           float* time, int32_t nsteps, void* stream) {
+  g_last_error.clear();
+  int rc = validate(1, params);
+  if (rc) return rc;
+  if (n < 0) return fail(FLUID_HIP_EINVAL, "negative particle count");
+  if (n > FLUID_HIP_NOALLOC_MAX_PARTICLES)
+    return fail(FLUID_HIP_EINVAL, "fluid_hip_step_noalloc takes at most " +
+                                      std::to_string(FLUID_HIP_NOALLOC_MAX_PARTICLES) +
+                                      " particles (its fixed scratch array is " +
+                                      std::to_string(kNoAllocBytes >> 20) + " MiB per GPU)");
+  if (nsteps < 0) return fail(FLUID_HIP_EINVAL, "nsteps < 0");
+  if (n > 0 && (!pos || !vel)) return fail(FLUID_HIP_EINVAL, "pos/vel is null");
+  const float dt = params->dt;
+  if (n == 0 || nsteps == 0) {
+    if (time)
+      for (int s = 0; s < nsteps; ++s) *time += dt;  // core.cpp:122-125
+    return FLUID_HIP_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  unsigned char* base = nullptr;
+  if ((rc = noalloc_scratch(dev, &base))) return rc;
+
+  // A context on the stack whose every buffer lies in the scratch array;
+  // its vectors stay empty, so nothing is allocated on the host either.
+  Ctx c;
+  c.device = dev;
+  c.stream = static_cast<hipStream_t>(stream);
+  c.n = static_cast<int>(n);
+  c.n_scenes = 1;
+  compute_consts_of(c, params, 1);
+  if (!noalloc_layout(c, base, kNoAllocBytes))
+    return fail(FLUID_HIP_ECAPACITY, "buffers do not fit the scratch array");
+
+  k_noalloc_init<<<1, 64, 0, c.stream>>>(c.n, 256 * c.radix_blocks, scene_table(*params), c.seg,
+                                         c.planes, c.n_dev, c.radix_len, c.bbox, c.flags,
+                                         c.steps_started);
+  HIP_TRY(hipGetLastError(), "scratch set-up");
+  k_zero_ints<<<1024, 256, 0, c.stream>>>(c.count, c.cell_cap);
+  HIP_TRY(hipGetLastError(), "cell table clear");
+  HIP_TRY(fluidhip::enqueue_upload(c, pos, vel), "upload kernel");
+  for (int s = 0; s < nsteps; ++s) {
+    const hipError_t e = fluidhip::enqueue_substep(c);
+    if (e != hipSuccess) return hip_fail(e, "substep launch");
+  }
+  DevStatus st{};
+  if ((rc = read_status(c, st))) return rc;
+  if (st.flags & fluidhip::kFlagRange)
+    return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
+  if (st.flags & fluidhip::kFlagCells)
+    return fail(FLUID_HIP_ECAPACITY, "cell table of " + std::to_string(c.cell_cap) +
+                                         " cells exceeded (needed " +
+                                         std::to_string(st.need_cells) +
+                                         "): the scene is too spread out for the scratch array");
+  if (st.flags & fluidhip::kFlagNeighbors)
+    return fail(FLUID_HIP_ECAPACITY, "neighbor lists of " + std::to_string(c.nbr_cap) +
+                                         " words exceeded: too many neighbors for the scratch "
+                                         "array");
+  HIP_TRY(fluidhip::enqueue_download(c, pos, vel), "download kernel");
+  HIP_TRY(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if (time)
+    for (int s = 0; s < nsteps; ++s) *time += dt;  // core.cpp:614
   return FLUID_HIP_OK;
 }
 

@@ -94,8 +94,13 @@ _OPTIONAL_SIGS: dict = {
     "fluid_hip_forward_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "fluid_hip_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
+    "fluid_hip_step_noalloc": (C.c_int, [C.POINTER(FluidHipParams), C.c_int64, C.c_void_p,
+                                         C.c_void_p, C.POINTER(C.c_float), C.c_int32,
+                                         C.c_void_p]),
 }
 
+NOALLOC_MAX_PARTICLES = 32768
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 def exported_symbols():
@@ -345,6 +350,19 @@ class HipSolver:
         cnt = C.c_int64(0)
         _check(self.lib.fluid_hip_profile_read(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
         return float(ms.value), int(cnt.value)
+
+
+def step_noalloc(params: Params, n: int, pos_ptr: int, vel_ptr: int, time: float = 0.0,
+                 nsteps: int = 1, stream: int | None = None) -> float:
+    """fluid_hip_step_noalloc: `nsteps` substeps of one scene on DEVICE SoA
+    arrays (3*n floats each, original order) in place, with every buffer in
+    the library's fixed per-GPU scratch array (no device allocation).
+    Returns the advanced time."""
+    t = C.c_float(time)
+    _check(load_library().fluid_hip_step_noalloc(
+        C.byref(pack_params(params)), int(n), C.c_void_p(pos_ptr), C.c_void_p(vel_ptr),
+        C.byref(t), int(nsteps), C.c_void_p(stream) if stream else None))
+    return float(t.value)
 
 
 def hip_step(params: Params, state: State) -> None:

@@ -268,6 +268,31 @@ int fluid_hip_forward_record(fluid_hip_ctx* ctx, int32_t nsteps, int32_t checkpo
 int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel, float* dpos0,
                        float* dvel0, float* dgravity);
 
+/* ---- stepping without device allocation ---------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): a stateless
+ * variant of create + upload + fluid_hip_step + download + destroy for one
+ * scene that allocates nothing.  Every buffer of the substep lives in one
+ * static device array of FLUID_HIP_NOALLOC_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends; the cell table and the
+ * neighbor lists get fixed shares of it and never grow (a scene that needs
+ * more fails with FLUID_HIP_ECAPACITY, leaving pos/vel unchanged).
+ *   params  one scene's parameters (validated as by fluid_hip_create)
+ *   n       particles, 0 <= n <= FLUID_HIP_NOALLOC_MAX_PARTICLES (else EINVAL)
+ *   pos,vel DEVICE SoA blocks of 3*n floats in original order, advanced in
+ *           place by `nsteps` substeps
+ *   time    host float advanced by nsteps * dt (may be NULL)
+ *   stream  hipStream_t to queue on (NULL: the null stream); synchronized
+ *           once at the end to check the device flags
+ * Results are bitwise those of fluid_hip_step.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads, other
+ * streams) race and must be serialized by the caller. */
+#define FLUID_HIP_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define FLUID_HIP_NOALLOC_MAX_PARTICLES 32768
+int fluid_hip_step_noalloc(const fluid_hip_params* params, int64_t n, float* pos, float* vel,
+                           float* time, int32_t nsteps, void* stream);
+
 /* ---- timing -------------------------------------------------------------- */
 
 /* Per-kernel device time (HIP events on the context's stream) of the steps
