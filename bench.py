@@ -46,7 +46,7 @@ STEP_BYTES_ALL_FLAGS = 480.0   # whole substep, all flags, 4 iterations
 FLOPS_PER_PAIR = {"lambda": 33.0, "delta": 30.0, "xsph": 25.0, "vorticity_omega": 31.0,
                   "vorticity_apply": 23.0}
 FP32_PEAK_TFLOPS = 157.3
-KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "runs",
+KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "tiles", "runs",
            "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply"]
 
 
@@ -545,7 +545,7 @@ def main():
                     help="skip the one-thread step of the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--xcd-remap", type=int, default=0)
     ap.add_argument("--brick-edge", type=int, default=0, help="cell table brick edge (0: 4)")
     ap.add_argument("--dims", type=str, default="128,128,64")
     ap.add_argument("--workload", choices=["million_block", "ensemble", "adjoint"],

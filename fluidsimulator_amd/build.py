@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfluid_hip.so")
-SOURCES = ["pbf_kernels.hip", "pbf_adjoint.hip", "pbf_capi.hip"]
+SOURCES = ["pbf_kernels.hip", "pbf_tiles.hip", "pbf_adjoint.hip", "pbf_capi.hip"]
 ARCH = os.environ.get("FLUID_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -48,6 +48,7 @@ def hipcc() -> str:
 APPDIR = os.path.join(HERE, "app")
 BINDIR = os.path.join(HERE, "bin")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+CHECKDIR = os.path.join(os.path.dirname(HERE), "tests", "native")
 # host tools over the C ABI (plain C++, no HIP headers): the app with the
 # reference's command line, the parity harness's diff tool and the
 # inverse-problem driver
@@ -66,6 +67,25 @@ def build_tools(verbose: bool = False) -> list:
         cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra",
                "-Wno-unused-parameter", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{APPDIR}", src, "-o", exe + ".tmp",
                f"-L{LIBDIR}", "-lfluid_hip", "-pthread", "-Wl,-rpath,$ORIGIN/../lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
+def build_checks(verbose: bool = False) -> list:
+    """Compile the device-side check programs of tests/native (one .hip file
+    each, built next to its source)."""
+    out = []
+    if not os.path.isdir(CHECKDIR):
+        return out
+    for name in sorted(os.listdir(CHECKDIR)):
+        if not name.endswith(".hip"):
+            continue
+        exe = os.path.join(CHECKDIR, name[:-4])
+        cmd = [hipcc(), *HIPCC_FLAGS, f"-I{CSRC}", os.path.join(CHECKDIR, name), "-o", exe + ".tmp"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
@@ -98,6 +118,7 @@ def build(verbose: bool = False) -> str:
     for o in objs:
         os.remove(o)
     build_tools(verbose)
+    build_checks(verbose)
     return LIB
 
 

@@ -185,6 +185,7 @@ void compute_consts_of(Ctx& c, const fluid_hip_params* params, int n_scenes) {
   k.runs = (c.wave_mode == 2 && !c.record_diag) ? 1 : 0;
   k.full_lists = c.record_diag ? 1 : 0;
   k.count_only = 0;
+  k.tiles = (k.runs && c.brick_shift == 2 && c.tile_gather != 2 && p.solver_iterations > 0) ? 1 : 0;
   c.consts = k;
   c.iterations = p.solver_iterations;
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423
@@ -262,6 +263,13 @@ void free_all(Ctx& c) {
     dfree(*p);
   dfree(c.slot_cell);
   dfree(c.grid);
+  dfree(c.in_tile);
+  dfree(c.tile_count);
+  dfree(c.tile_rec);
+  dfree(c.tile_seg);
+  dfree(c.tile_src);
+  dfree(c.tile_list);
+  dfree(c.tile_cnt);
   dfree(c.noff);
   dfree(c.nwords);
   dfree(c.ncount);
@@ -653,6 +661,14 @@ bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
   take(c.planes, 1);
   take(c.flags, 1);
   take(c.steps_started, 1);
+  c.tile_cap = static_cast<long long>(n / 16 + 256);
+  take(c.in_tile, n);
+  take(c.tile_count, 1);
+  take(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
+  take(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
+  take(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
+  take(c.tile_list, static_cast<size_t>(fluidhip::kTileList) * n);
+  take(c.tile_cnt, n);
   if (!ok) return false;
   off = (off + 255) & ~static_cast<size_t>(255);
   const size_t rest = bytes - off;
@@ -743,6 +759,8 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
     return fail(FLUID_HIP_EINVAL, "wave_mode must be 0..3");
   const int bshift = brick_shift_of(cfg.brick_edge);
   if (bshift < 0) return fail(FLUID_HIP_EINVAL, "brick_edge must be 0, 1, 2, 4 or 8");
+  if (cfg.tile_gather < 0 || cfg.tile_gather > 2)
+    return fail(FLUID_HIP_EINVAL, "tile_gather must be 0, 1 or 2");
 
   auto* h = new (std::nothrow) fluid_hip_ctx();
   if (!h) return fail(FLUID_HIP_EINVAL, "out of host memory");
@@ -753,6 +771,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   c.record_diag = cfg.record_diag != 0;
   c.wave_mode = cfg.wave_mode ? cfg.wave_mode : 2;
   c.brick_shift = bshift;
+  c.tile_gather = cfg.tile_gather;
   c.n = static_cast<int>(total);
   c.n_scenes = n_scenes;
   c.params.assign(params, params + n_scenes);
@@ -836,6 +855,15 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.flags, 1);
   ALLOC(c.steps_started, 1);
   ALLOC(c.stage, 6 * n);
+  // brick tiles: beyond this capacity bricks take the per-lane passes
+  c.tile_cap = static_cast<long long>(n / 16 + 256);
+  ALLOC(c.in_tile, n);
+  ALLOC(c.tile_count, 1);
+  ALLOC(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
+  ALLOC(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
+  ALLOC(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
+  ALLOC(c.tile_list, static_cast<size_t>(fluidhip::kTileList) * n);
+  ALLOC(c.tile_cnt, n);
 #undef ALLOC
   e = hipMemcpy(c.n_dev, &c.n, sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(particle count)"));
@@ -1146,11 +1174,56 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
   HIP_TRY(hipMemcpy(pos.data(), c.posA, sizeof(float4) * n, hipMemcpyDeviceToHost), "read pos");
   HIP_TRY(hipMemcpy(off.data(), c.noff, sizeof(long long) * n, hipMemcpyDeviceToHost), "read offsets");
   HIP_TRY(hipMemcpy(nw.data(), c.nwords, sizeof(int) * n, hipMemcpyDeviceToHost), "read lengths");
-  for (int x : nw)
-    if (x < 0)
+  // particles of LDS tiles keep a candidate mask instead of a list: decode it
+  // with the tile's halo layout (the candidates are the 27 cells in (dz, dy,
+  // dx) order, 9 contiguous runs of three cells of the halo, as in
+  // pbf_tiles.hip)
+  const bool tiled = c.consts.tiles && c.last_runs && c.in_tile != nullptr;
+  std::vector<int> in_tile, tseg, tsrc, tcnt;
+  std::vector<uint16_t> tlist;
+  if (tiled) {
+    in_tile.resize(n);
+    HIP_TRY(hipMemcpy(in_tile.data(), c.in_tile, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "read tiles");
+  }
+  bool any_tile = false;
+  for (size_t s = 0; s < n; ++s) {
+    if (nw[s] >= 0) continue;
+    if (!tiled || in_tile[s] <= 0)
       return fail(FLUID_HIP_ESTATE,
                   "the last step stored no list for some particles; create the context "
                   "with record_diag for complete lists");
+    any_tile = true;
+  }
+  if (any_tile) {
+    const size_t cap = static_cast<size_t>(c.tile_cap);
+    tseg.resize(cap * 217);
+    tsrc.resize(cap * 216);
+    tlist.resize(static_cast<size_t>(fluidhip::kTileList) * n);
+    tcnt.resize(n);
+    HIP_TRY(hipMemcpy(tseg.data(), c.tile_seg, sizeof(int) * tseg.size(), hipMemcpyDeviceToHost),
+            "read tiles");
+    HIP_TRY(hipMemcpy(tsrc.data(), c.tile_src, sizeof(int) * tsrc.size(), hipMemcpyDeviceToHost),
+            "read tiles");
+    HIP_TRY(hipMemcpy(tlist.data(), c.tile_list, sizeof(uint16_t) * tlist.size(),
+                      hipMemcpyDeviceToHost),
+            "read tile lists");
+    HIP_TRY(hipMemcpy(tcnt.data(), c.tile_cnt, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "read tile lists");
+  }
+  // neighbor slots of tile slot s, in list order: entry r is an LDS record of
+  // the tile's halo, the halo cells' records laid out by tile_seg (first
+  // record of each cell) and tile_src (its first slot)
+  auto tile_neighbors = [&](size_t s, std::vector<int>& outs) {
+    const size_t i = static_cast<size_t>(in_tile[s] - 1);
+    const int* seg = tseg.data() + i * 217;
+    const int* src = tsrc.data() + i * 216;
+    for (int k = 0; k < tcnt[s]; ++k) {
+      const int r = tlist[static_cast<size_t>(k) * n + s];
+      const int cc = static_cast<int>(std::upper_bound(seg, seg + 216, r) - seg) - 1;
+      outs.push_back(src[cc] + (r - seg[cc]));
+    }
+  };
   unsigned long long used = 0;
   HIP_TRY(hipMemcpy(&used, c.nbr_total, sizeof(used), hipMemcpyDeviceToHost), "read total");
   used = std::min<unsigned long long>(used, static_cast<unsigned long long>(c.nbr_cap));
@@ -1165,8 +1238,18 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
     slot_of[id] = static_cast<int>(s);
   }
   int64_t k = 0;
+  std::vector<int> tn;
   for (size_t id = 0; id < n; ++id) {
     const int s = slot_of[id];
+    if (nw[s] < 0) {
+      tn.clear();
+      tile_neighbors(static_cast<size_t>(s), tn);
+      for (int t : tn) {
+        if (out && k < cap) out[k] = ids[t];
+        ++k;
+      }
+      continue;
+    }
     const int* L = list.data() + off[s];
     for (int w = 0; w < nw[s];) {
       const int e = L[w++];
@@ -1210,8 +1293,8 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
   if (rc) return rc;
   Ctx& c = ctx->c;
   if (!out || n < 0) return fail(FLUID_HIP_EINVAL, "bad output");
-  int64_t v[8] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events, tape_bytes(c),
-                  c.adj ? c.adj->recomputed : 0};
+  int64_t v[10] = {0, 0, 0, c.nbr_cap, c.cell_cap, c.grow_events, tape_bytes(c),
+                   c.adj ? c.adj->recomputed : 0, 0, 0};
   if (c.stepped && c.n > 0) {
     if (c.lists_partial) {
       // cooperative particles stored no list: count their pairs again
@@ -1234,8 +1317,23 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
     int cells = 0;
     HIP_TRY(hipMemcpy(&cells, c.total_cells, sizeof(int), hipMemcpyDeviceToHost), "read cells");
     v[2] = cells;
+    if (c.consts.tiles && c.last_runs && c.tile_count) {
+      int tiles = 0;
+      HIP_TRY(hipMemcpy(&tiles, c.tile_count, sizeof(int), hipMemcpyDeviceToHost), "read tiles");
+      tiles = static_cast<int>(std::min<int64_t>(tiles, c.tile_cap));
+      std::vector<int4> rec(2 * static_cast<size_t>(tiles));
+      if (tiles)
+        HIP_TRY(hipMemcpy(rec.data(), c.tile_rec, sizeof(int4) * rec.size(),
+                          hipMemcpyDeviceToHost),
+                "read tiles");
+      for (int i = 0; i < tiles; ++i)
+        if (rec[2 * i + 1].w >= 0) {  // live: not handed back to the per-lane passes
+          ++v[8];
+          v[9] += rec[2 * i].y - rec[2 * i].x;
+        }
+    }
   }
-  for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+  for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
   return FLUID_HIP_OK;
 }
 
@@ -1273,9 +1371,9 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
   if (trial_steps < 1) return fail(FLUID_HIP_EINVAL, "trial_steps < 1");
   if (knobs == 0) knobs = FLUID_HIP_TUNE_DEFAULT;
   struct Knobs {
-    int block, remap, wave, bshift;
+    int block, remap, wave, bshift, tiles;
   };
-  const Knobs start{c.block, c.xcd_remap, c.wave_mode, c.brick_shift};
+  const Knobs start{c.block, c.xcd_remap, c.wave_mode, c.brick_shift, c.tile_gather == 2 ? 2 : 1};
   const size_t n = static_cast<size_t>(c.n);
   const size_t soa = 3 * n;
   invalidate_record(c);
@@ -1299,6 +1397,7 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
     c.xcd_remap = k.remap;
     c.wave_mode = k.wave;
     c.brick_shift = k.bshift;
+    c.tile_gather = k.tiles;
     compute_consts(c);
     drop_graph(c);
   };
@@ -1345,16 +1444,25 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
     const int waves[] = {2, 3, 1};
     const int shifts[] = {0, 1, 2, 3};
     const int remaps[] = {1, 0};
-    const int knob_bit[4] = {FLUID_HIP_TUNE_WAVE_MODE, FLUID_HIP_TUNE_BLOCK,
-                             FLUID_HIP_TUNE_BRICK, FLUID_HIP_TUNE_XCD};
+    const int tiles[] = {1, 2};
+    const int knob_bit[5] = {FLUID_HIP_TUNE_WAVE_MODE, FLUID_HIP_TUNE_BLOCK,
+                             FLUID_HIP_TUNE_BRICK, FLUID_HIP_TUNE_XCD, FLUID_HIP_TUNE_TILES};
     for (int pass = 0; pass < 2; ++pass) {
-      for (int knob = 0; knob < 4; ++knob) {
+      for (int knob = 0; knob < 5; ++knob) {
         if (!(knobs & knob_bit[knob])) continue;
-        const int* vals = knob == 0 ? waves : knob == 1 ? blocks : knob == 2 ? shifts : remaps;
+        const int* vals = knob == 0   ? waves
+                          : knob == 1 ? blocks
+                          : knob == 2 ? shifts
+                          : knob == 3 ? remaps
+                                      : tiles;
         const int nv = knob == 0 ? 3 : knob == 1 ? 5 : knob == 2 ? 4 : 2;
         for (int i = 0; i < nv; ++i) {
           Knobs k = cur;
-          int& slot = knob == 0 ? k.wave : knob == 1 ? k.block : knob == 2 ? k.bshift : k.remap;
+          int& slot = knob == 0   ? k.wave
+                      : knob == 1 ? k.block
+                      : knob == 2 ? k.bshift
+                      : knob == 3 ? k.remap
+                                  : k.tiles;
           if (slot == vals[i]) continue;
           slot = vals[i];
           if ((r = trial(k, ms, out))) return r;
@@ -1380,6 +1488,7 @@ int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
       best->xcd_remap = cur.remap;
       best->wave_mode = cur.wave;
       best->brick_edge = 1 << cur.bshift;
+      best->tile_gather = cur.tiles;
       best->cell_capacity = c.cell_cap;
       best->neighbor_capacity = c.nbr_cap;
       best->lane_capacity = c.lane_cap;

@@ -13,6 +13,23 @@
 
 namespace fluidhip {
 
+// Brick tiles of the LDS-tiled gather passes (pbf_tiles.hip).
+constexpr int kTileMaxHalo = 2048;  // particles of a tile's 6^3-cell halo held in LDS
+constexpr int kTileList = 128;      // neighbors of a tile particle, at most (else per-lane)
+constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
+
+// What the tile kernels read besides their pass's arrays.
+struct TileArgs {
+  const uint64_t* slot_cell;
+  const int* tile_count;   // tiles found this substep (device)
+  int tile_cap;
+  const int4* tile_rec;    // per tile: {a, e, halo records, scene}, {first cell x, y, z, brick}
+  const int* tile_seg;     // per tile: 217 LDS offsets of the halo cells
+  const int* tile_src;     // per tile: 216 first slots of the halo cells
+  const uint16_t* tile_list;  // kTileList x n: neighbors' LDS records
+  const int* tile_cnt;        // n: neighbors of a tile particle
+};
+
 struct Prof {
   struct Pending {
     const char* name;
@@ -47,6 +64,7 @@ struct Ctx {
   bool record_diag = false;
   int wave_mode = 2;
   int brick_shift = 2;  // log2 of the cells per brick edge of the cell table
+  int tile_gather = 0;  // 0 default (on), 1 on, 2 off: LDS-tiled passes for sparse bricks
 
   // state (A = committed state between substeps)
   float4* posA = nullptr;  // xyz, w = original id bits
@@ -99,6 +117,16 @@ struct Ctx {
   int* lane_rep = nullptr;   // lane_cap: the run head slot of each lane, -1 for padding
   int* lane_info = nullptr;  // 3: packed lanes, first aligned lane, end of aligned lanes
   long long lane_cap = 0;    // lanes the passes launch with runs: 3n + 64
+
+  // brick tiles (sparse bricks whose passes run through LDS)
+  int* in_tile = nullptr;      // per slot: 1 + its tile's index this substep, 0 outside tiles
+  int* tile_count = nullptr;   // tiles found this substep
+  int4* tile_rec = nullptr;    // 2 tile_cap: tile records
+  int* tile_seg = nullptr;     // tile_cap x 217: LDS offset of each halo cell
+  int* tile_src = nullptr;     // tile_cap x 216: first slot of each halo cell
+  uint16_t* tile_list = nullptr;  // kTileList x n: neighbors' LDS records, entry-major
+  int* tile_cnt = nullptr;        // n: neighbors of a tile particle
+  long long tile_cap = 0;
 
   // neighbors
   long long* noff = nullptr;  // first word of each slot's list
@@ -223,5 +251,17 @@ hipError_t enqueue_reset_flags(Ctx& c);
 // cooperative particles are not stored) into c.pair_total.
 hipError_t enqueue_count_pairs(Ctx& c);
 hipError_t enqueue_init_grid(Ctx& c);
+
+// The tile kernels (pbf_tiles.hip).
+hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k);
+hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
+                                  const float4* p0);
+hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
+                               int write_rho);
+hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,
+                              float4* Pn, bool last, bool rest, float4* vout);
+hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest);
+hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta);
+hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest);
 
 }  // namespace fluidhip

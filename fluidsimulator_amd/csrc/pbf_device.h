@@ -57,6 +57,7 @@ struct StepConsts {
   int full_lists;   // 1: cooperative particles get stored lists too (otherwise
                     //    their passes retest the list's distance predicate)
   int count_only;   // 1: the neighbor walk only counts pairs (statistics)
+  int tiles;        // 1: sparse 4x4x4-cell bricks run the LDS-tiled passes
 };
 
 struct ScenePlanes {
@@ -88,6 +89,27 @@ __device__ __forceinline__ float poly6(float r2, const StepConsts& c) {
   return c.w_coeff * diff3;
 }
 
+// The correctly rounded square root (IEEE sqrt, as sqrtf under
+// -fhip-fp32-correctly-rounded-divide-sqrt).  For finite x >= 2^-96 the
+// hardware estimate s is corrected by the residuals x - s*s' of its float
+// neighbours s' -- the same correction the compiler emits for sqrtf, whose
+// other parts (scaling tiny inputs, passing 0 and inf through) do nothing
+// there, so both give the same bits; other x go through sqrtf itself.  The
+// gather passes call it once per neighbor, where those parts cost a third of
+// the square root.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (x >= 0x1p-96f && x <= 0x1.fffffep+127f) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s) - 1);
+    const float up = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = fmaf(-dn, s, x);
+    const float rup = fmaf(-up, s, x);
+    const float r = rdn <= 0.0f ? dn : s;
+    return rup > 0.0f ? up : r;
+  }
+  return sqrtf(x);
+}
+
 __device__ __forceinline__ float spiky(float r, const StepConsts& c) {
   // core.cpp:48-57 with the coefficient hoisted.
   if (r > c.h) return 0.0f;
@@ -112,7 +134,7 @@ __device__ __forceinline__ float pow_ratio(float ratio, int n) {
 // only the others pay for the IEEE square root.
 __device__ __forceinline__ float pair_spiky(float r2, const StepConsts& c) {
   if (r2 < c.min_r2) return c.f_min;
-  return spiky(sqrtf(r2), c);
+  return spiky(sqrt_rn(r2), c);
 }
 
 // The neighbor predicate: candidate q is a neighbor of a particle at p when

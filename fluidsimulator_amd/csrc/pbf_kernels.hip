@@ -269,7 +269,8 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
                                                     unsigned* __restrict__ flags,
                                                     unsigned long long* __restrict__ nbr_total,
                                                     unsigned long long* __restrict__ need_cells,
-                                                    int* __restrict__ steps_started) {
+                                                    int* __restrict__ steps_started,
+                                                    int* __restrict__ tile_count) {
   __shared__ long long sh[257];
   __shared__ long long carry;
   // After a failed substep the counters keep what the host needs to size
@@ -350,6 +351,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
   }
   if (threadIdx.x == 0 && !was_fatal) {
     *nbr_total = 0ull;
+    if (tile_count) *tile_count = 0;
     if (carry > cell_cap) {
       atomicOr(flags, kFlagCells);
       *total_cells = 0;
@@ -754,6 +756,15 @@ __device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellVie
   return L;
 }
 
+// With runs the passes launch over the lane map's capacity; workgroups past
+// the lanes in use ([0, max(lane_info[0], lane_info[2]))) have nothing to do
+// and return at once (uniform over the workgroup).
+__device__ __forceinline__ bool past_lanes(const StepConsts& c, const CellView& cv) {
+  if (!c.runs) return false;
+  const int first = tile_index(c.xcd_remap) * static_cast<int>(blockDim.x);
+  return first >= max(cv.lane_info[0], cv.lane_info[2]);
+}
+
 // Wavefront-uniform description of the group led by `leader`.
 struct Group {
   int rx, ry, rz;
@@ -1017,6 +1028,7 @@ __global__ __launch_bounds__(1024) void k_neighbors(
   __shared__ int abort_sh;
   // Other workgroups of this launch may raise the neighbor flag at any time:
   // read it once per workgroup so the barriers below stay uniform.
+  if (past_lanes(c, cv)) return;
   if (threadIdx.x == 0) abort_sh = fatal(flags) ? 1 : 0;
   __syncthreads();
   if (abort_sh) return;
@@ -1476,7 +1488,7 @@ __global__ __launch_bounds__(1024) void k_lambda(StepConsts c, float4* __restric
                                                  float* __restrict__ rho_out, int write_rho,
                                                  float4* __restrict__ sq_out,
                                                  const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
+  if (fatal(flags) || past_lanes(c, cv)) return;
   const LaneCell L =
       lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
                 nv.nw);
@@ -1589,7 +1601,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void 
     float4* __restrict__ posA, float4* __restrict__ vout, const float* __restrict__ rho,
     unsigned* __restrict__ pmask, unsigned* __restrict__ rmask,
     const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
+  if (fatal(flags) || past_lanes(c, cv)) return;
   const LaneCell L =
       lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
                 nv.nw);
@@ -1711,7 +1723,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void 
                                                CellView cv, const ScenePlanes* __restrict__ sp,
                                                unsigned* __restrict__ rmask,
                                                const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
+  if (fatal(flags) || past_lanes(c, cv)) return;
   const LaneCell L =
       lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
                 nv.nw);
@@ -1802,7 +1814,7 @@ __global__ __launch_bounds__(1024) void k_vort_omega(StepConsts c, const float4*
                                                      float4* __restrict__ omega, NbrView nv,
                                                      CellView cv,
                                                      const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
+  if (fatal(flags) || past_lanes(c, cv)) return;
   const LaneCell L =
       lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
                 nv.nw);
@@ -1883,7 +1895,7 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
                                                      CellView cv,
                                                      const ScenePlanes* __restrict__ sp,
                                                      const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
+  if (fatal(flags) || past_lanes(c, cv)) return;
   const LaneCell L =
       lane_cell(c, cv, tile_index(c.xcd_remap) * blockDim.x + threadIdx.x, nv.count,
                 nv.nw);
@@ -2075,6 +2087,7 @@ __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __
                                                      const int* __restrict__ seg,
                                                      int* __restrict__ lanes1,
                                                      int* __restrict__ lanes2,
+                                                     const int* __restrict__ in_tile,
                                                      const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2082,6 +2095,11 @@ __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __
   if (j >= C) return;
   const int a = first[j];
   const int b = j + 1 < C ? first[j + 1] : c.n;
+  if (in_tile && in_tile[a]) {  // the tile kernels compute this cell
+    lanes1[j] = 0;
+    lanes2[j] = 0;
+    return;
+  }
   const int R = run_no[b - 1] - run_no[a] + 1;
   bool al = aligned_cell(c, a, b);
   if (!al && c.coop && c.n <= (1 << 24)) {
@@ -2113,6 +2131,7 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
                                                    const int* __restrict__ cell_no,
                                                    const int* __restrict__ first,
                                                    const int* __restrict__ excl,
+                                                   const int* __restrict__ lanes1,
                                                    const int* __restrict__ lanes2,
                                                    const int* __restrict__ off1,
                                                    const int* __restrict__ off2,
@@ -2120,6 +2139,7 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
                                                    long long lane_cap,
                                                    int* __restrict__ lane_rep,
                                                    int* __restrict__ lane_info,
+                                                   const int* __restrict__ in_tile,
                                                    const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2138,7 +2158,9 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
   }
   if (s > 0 && run_no[s] == run_no[s - 1]) return;  // not a run head
   if (!fits) {
-    lane_rep[run_no[s]] = s;
+    // the tile kernels compute tile slots (the eta pass updates vel in
+    // place, so no slot may be computed twice): their lanes stay idle
+    lane_rep[run_no[s]] = (in_tile && in_tile[s]) ? -1 : s;
     return;
   }
   const int j = cell_no[s];
@@ -2146,7 +2168,7 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
   const int b = j + 1 < C ? first[j + 1] : c.n;
   const int rr = run_no[s] - run_no[a];
   if (lanes2[j] == 0) {
-    lane_rep[total2 + off1[j] + rr] = s;
+    if (lanes1[j] != 0) lane_rep[total2 + off1[j] + rr] = s;  // 0: a tile's cell
     return;
   }
   const int lane0 = off2[j];
@@ -2256,8 +2278,14 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (rec) {
     k.runs = 0;
     k.full_lists = 1;
+    k.tiles = 0;
   }
   const bool runs = k.runs != 0;
+  // sparse bricks through the LDS-tiled passes (needs the lane map of runs)
+  const bool tiles = runs && k.tiles != 0 && c.tile_list != nullptr;
+  if (!tiles) k.tiles = 0;
+  const TileArgs ta{c.slot_cell, c.tile_count, static_cast<int>(c.tile_cap), c.tile_rec,
+                    c.tile_seg,  c.tile_src,   c.tile_list, c.tile_cnt};
   const CellView cv{c.slot_cell, c.slot_key, c.grid,      c.start,
                     c.seg,       c.lane_rep, c.lane_info, runs ? c.run_end : nullptr,
                     PK(0)};
@@ -2288,7 +2316,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   FH_TRY(L("grid_setup", [&] {
     k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.seg, c.tile_bbox, bs, c.bbox, c.grid,
                                     c.total_cells, c.cell_cap, c.brick_shift,
-                                    c.flags, c.nbr_total, c.need_cells, c.steps_started);
+                                    c.flags, c.nbr_total, c.need_cells, c.steps_started,
+                                    tiles ? c.tile_count : nullptr);
   }));
   FH_TRY(L("cell_count", [&] {
     k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.flags);
@@ -2325,6 +2354,13 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                  c.posA, c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
+  // tiles and their neighbor lists come before the lane map: a tile with a
+  // particle of more than kTileList neighbors is handed back to the lanes
+  if (tiles)
+    FH_TRY(L("tiles", [&] {
+      (void)enqueue_tile_classify(c, k);
+      (void)enqueue_tile_neighbors(c, k, ta, PK(0));
+    }));
   if (runs)
     FH_TRY(L("runs", [&] {
       k_run_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.posB, c.velB, c.slot_key, c.run_no,
@@ -2347,7 +2383,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                                          c.flags);
       k_cell_lanes<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_excl, c.cell_first, c.run_no,
                                                           c.slot_cell, c.grid, c.start, c.seg,
-                                                          c.lanes1, c.lanes2, c.flags);
+                                                          c.lanes1, c.lanes2,
+                                                          tiles ? c.in_tile : nullptr, c.flags);
       for (int q = 0; q < 2; ++q) {
         int* lanes = q == 0 ? c.lanes1 : c.lanes2;
         int* off = q == 0 ? c.off1 : c.off2;
@@ -2358,8 +2395,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                                    0, c.flags);
       }
       k_lane_map<<<blocks_for(c.n, 256), 256, 0, st>>>(
-          k, c.run_no, c.cell_no, c.cell_first, c.cell_excl, c.lanes2, c.off1, c.off2,
-          c.run_excl + c.n, c.lane_cap, c.lane_rep, c.lane_info, c.flags);
+          k, c.run_no, c.cell_no, c.cell_first, c.cell_excl, c.lanes1, c.lanes2, c.off1, c.off2,
+          c.run_excl + c.n, c.lane_cap, c.lane_rep, c.lane_info, tiles ? c.in_tile : nullptr,
+          c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
@@ -2380,10 +2418,12 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     unsigned* pm = rec ? rec->M[it] : nullptr;
     unsigned* rm = rec ? rec->RM : nullptr;
     FH_TRY(L("lambda", [&] {
+      if (tiles) (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0);
       k_lambda<<<nl, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
       fill(P, nullptr);
     }));
     FH_TRY(L("delta", [&] {
+      if (tiles) (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1);
       if (!last)
         k_delta<true, false, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
@@ -2421,6 +2461,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (c.xsph) {
     const bool rest = c.restitution_any && !c.vort;
     FH_TRY(L("xsph", [&] {
+      if (tiles) (void)enqueue_tile_xsph(c, k, ta, rest);
       if (rest)
         k_xsph<true><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                         c.planes, rec ? rec->RM : nullptr, c.flags);
@@ -2432,10 +2473,12 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
+      if (tiles) (void)enqueue_tile_vort_omega(c, k, ta);
       k_vort_omega<<<nl, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
       fill(c.omega, nullptr);
     }));
     FH_TRY(L("vorticity_apply", [&] {
+      if (tiles) (void)enqueue_tile_vort_apply(c, k, ta, c.restitution_any);
       if (c.restitution_any)
         k_vort_apply<true><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                               c.planes, c.flags);

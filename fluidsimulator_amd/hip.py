@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfluid_hip.so")
 MAX_PLANES = 8
 
 OK, EINVAL, ENODEV, EHIP, ECAPACITY, ERANGE, EUNSUPPORTED, ESTATE = 0, -1, -2, -3, -4, -5, -6, -7
-TUNE_WAVE_MODE, TUNE_BLOCK, TUNE_BRICK, TUNE_XCD = 1, 2, 4, 8
+TUNE_WAVE_MODE, TUNE_BLOCK, TUNE_BRICK, TUNE_XCD, TUNE_TILES = 1, 2, 4, 8, 16
 
 
 class FluidHipParams(C.Structure):
@@ -46,6 +46,7 @@ class FluidHipConfig(C.Structure):
         ("cell_capacity", C.c_int64), ("neighbor_capacity", C.c_int64), ("device", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("record_diag", C.c_int32),
         ("wave_mode", C.c_int32), ("brick_edge", C.c_int32), ("lane_capacity", C.c_int64),
+        ("tile_gather", C.c_int32),
     ]
 
 
@@ -187,7 +188,7 @@ class HipSolver:
 
     def __init__(self, params, counts, *, cell_capacity=0, neighbor_capacity=0, block_size=256,
                  xcd_remap=True, record_diag=False, device=0, wave_mode=0, brick_edge=0,
-                 lane_capacity=0):
+                 lane_capacity=0, tile_gather=0):
         self.lib = load_library()
         plist = list(params) if isinstance(params, (list, tuple)) else [params]
         counts = [int(c) for c in counts]
@@ -203,7 +204,8 @@ class HipSolver:
         cnt = (C.c_int64 * self.n_scenes)(*counts)
         cfg = FluidHipConfig(int(cell_capacity), int(neighbor_capacity), int(device),
                              int(block_size), int(bool(xcd_remap)), int(bool(record_diag)),
-                             int(wave_mode), int(brick_edge), int(lane_capacity))
+                             int(wave_mode), int(brick_edge), int(lane_capacity),
+                             int(tile_gather))
         h = C.c_void_p()
         _check(self.lib.fluid_hip_create(C.byref(h), self.n_scenes, cnt, arr, C.byref(cfg)))
         self.h = h
@@ -294,12 +296,13 @@ class HipSolver:
     def stats(self):
         """Sizes of the last step: neighbor pairs, encoded list words, grid
         cells, list capacity (words), cell capacity, capacity growths; device
-        bytes of the rollout recording and substeps the last backward pass
-        recomputed."""
-        out = (C.c_int64 * 8)()
-        _check(self.lib.fluid_hip_get_stats(self.h, out, 8))
+        bytes of the rollout recording, substeps the last backward pass
+        recomputed, and the bricks the last substep ran as LDS tiles and
+        their particles."""
+        out = (C.c_int64 * 10)()
+        _check(self.lib.fluid_hip_get_stats(self.h, out, 10))
         keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events",
-                "record_bytes", "recomputed")
+                "record_bytes", "recomputed", "tiles", "tiled_particles")
         return dict(zip(keys, [int(x) for x in out]))
 
     def autotune(self, trial_steps=2, knobs=0):
@@ -313,6 +316,7 @@ class HipSolver:
                                            rep))
         return {"block_size": cfg.block_size, "xcd_remap": cfg.xcd_remap,
                 "wave_mode": cfg.wave_mode, "brick_edge": cfg.brick_edge,
+                "tile_gather": cfg.tile_gather,
                 "start_ms_per_step": rep[0], "best_ms_per_step": rep[1],
                 "candidates_timed": int(rep[2]), "candidates_rejected": int(rep[3])}
 

@@ -100,6 +100,9 @@ typedef struct fluid_hip_config {
                                  0 default (3n + 64), otherwise clamped to
                                  [n, 3n + 64]; a layout that does not fit
                                  packs one lane per run */
+  int32_t tile_gather;        /* passes of sparse 4x4x4-cell bricks through LDS tiles:
+                                 0 default (on), 1 on, 2 off; used with brick edge 4 and
+                                 gather mode 2 (results are the same either way) */
 } fluid_hip_config;
 
 typedef struct fluid_hip_ctx fluid_hip_ctx;
@@ -209,7 +212,9 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
  * (words), out[4] cell-table capacity, out[5] capacity growths since the
  * context was created, out[6] device bytes held by the rollout recording
  * (checkpoints and tapes), out[7] substeps the last backward pass
- * recomputed.  `n` entries are written (<= 8). */
+ * recomputed, out[8] bricks the last substep ran through the LDS-tiled
+ * passes, out[9] particles of those bricks.  `n` entries are written
+ * (<= 10). */
 int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
 
 /* ---- autotuning ---------------------------------------------------------- */
@@ -218,7 +223,9 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
  * context by timing, one knob at a time over two passes, among the knobs in
  * `knobs` (a mask of FLUID_HIP_TUNE_*; 0 selects FLUID_HIP_TUNE_DEFAULT):
  * the workgroup size (64..1024), the cell table's brick edge (1, 2, 4, 8),
- * the XCD-aware tile order and, on request, the gather mode of wavefronts
+ * the XCD-aware tile order, the LDS-tiled passes of sparse bricks (a
+ * kernel-structure choice: one persistent tile kernel per pass, or every
+ * particle through the per-lane kernels) and, on request, the gather mode of wavefronts
  * sharing a cell (wave_mode 1..3).  The gather mode is left out by default:
  * which mode is fastest depends on how far particles have piled up, which a
  * short trial at one state does not show.  Every candidate runs
@@ -233,7 +240,9 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
 #define FLUID_HIP_TUNE_BLOCK 2
 #define FLUID_HIP_TUNE_BRICK 4
 #define FLUID_HIP_TUNE_XCD 8
-#define FLUID_HIP_TUNE_DEFAULT (FLUID_HIP_TUNE_BLOCK | FLUID_HIP_TUNE_BRICK | FLUID_HIP_TUNE_XCD)
+#define FLUID_HIP_TUNE_TILES 16
+#define FLUID_HIP_TUNE_DEFAULT \
+  (FLUID_HIP_TUNE_BLOCK | FLUID_HIP_TUNE_BRICK | FLUID_HIP_TUNE_XCD | FLUID_HIP_TUNE_TILES)
 int fluid_hip_autotune(fluid_hip_ctx* ctx, int32_t trial_steps, int32_t knobs,
                        fluid_hip_config* best, double* report);
 
@@ -297,7 +306,7 @@ int fluid_hip_step_noalloc(const fluid_hip_params* params, int64_t n, float* pos
 
 /* Per-kernel device time (HIP events on the context's stream) of the steps
  * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_count",
- * "scan", "sort", "reorder", "runs", "neighbors", "lambda", "delta", "xsph",
+ * "scan", "sort", "reorder", "tiles", "runs", "neighbors", "lambda", "delta", "xsph",
  * "vorticity_omega", "vorticity_apply".  Launches of a substep that
  * overflowed a capacity, and of the substeps queued behind it, are not
  * counted (fluid_hip_step runs them again after growing the tables). */
