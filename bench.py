@@ -206,8 +206,7 @@ def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | No
             line = json.loads(f.read().strip().splitlines()[-1])
         cfg = line.get("config", {})
         run = {"dims": cfg.get("dims"), "warmup": line.get("warmup"), "steps": line.get("steps"),
-               "config": {k: cfg.get(k) for k in ("block", "xcd_remap", "wave_mode",
-                                                  "brick_edge", "graph")}}
+               "config": {k: cfg.get(k) for k in RUN_KEYS}}
         if steps is None:
             steps = int(line["steps"])
     if steps is None:
@@ -230,11 +229,19 @@ def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | No
     km = per_dispatch("pmc_l2", "TCC_MISS_sum", required=False)
 
     def window(d, key, per_step):
+        """Per-launch values of pass `key` over the timed window: the per-lane
+        kernel and, where the pass has one, its LDS-tile kernel (both run in
+        every launch of the pass) summed launch by launch."""
         if d is None:
             return []
-        name = ROCPROF_NAME[key]
-        vals = [v for _, v in sorted(x for k in d if name in k for x in d[k])]
-        return vals[-per_step * steps:]
+        out = None
+        for name in (ROCPROF_NAME[key], "k_tile_" + ROCPROF_NAME[key][2:]):
+            vals = [v for _, v in sorted(x for k in d if name in k for x in d[k])]
+            vals = vals[-per_step * steps:]
+            if not vals:
+                continue
+            out = vals if out is None else [a + b for a, b in zip(out, vals)]
+        return out or []
 
     res = {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]}, the timed window of "
                        "the bench command",
@@ -497,6 +504,31 @@ def bench_adjoint(args) -> dict:
     }
 
 
+TUNED_KEYS = ("block_size", "xcd_remap", "wave_mode", "brick_edge", "tile_gather")
+# launch configuration fields a counter summary must share with a bench line
+RUN_KEYS = ("block", "xcd_remap", "wave_mode", "brick_edge", "tile_gather", "graph")
+
+
+def tuned_config(args):
+    """Launch configuration of the timed region and where it came from: the
+    file --tuned names (auto: profiles/tuned_million_block_warmup<W>.json,
+    the tuner's choice for this warm-up, committed so that every run of the
+    same command launches the same kernels), else the library defaults with
+    --block / --xcd-remap / --brick-edge."""
+    path = args.tuned
+    if path == "auto":
+        path = os.path.join(ROOT, "profiles", f"tuned_million_block_warmup{args.warmup}.json")
+        if not os.path.exists(path):
+            path = "none"
+    if path != "none" and args.autotune == 0:
+        with open(path) as f:
+            rec = json.load(f)
+        launch = {k: int(rec["launch"][k]) for k in TUNED_KEYS}
+        return launch, os.path.relpath(path, ROOT)
+    return ({"block_size": args.block, "xcd_remap": int(args.xcd_remap), "wave_mode": 0,
+             "brick_edge": args.brick_edge, "tile_gather": 0}, "defaults")
+
+
 def read_kernels(s) -> dict:
     """Per-kernel device time of the substeps queued since profiling was
     switched on (HIP events on the context's stream)."""
@@ -554,9 +586,15 @@ def main():
                          "gradient of a rollout against the rollout (each its own JSON line)")
     ap.add_argument("--every", type=int, default=10,
                     help="checkpoint interval of the adjoint's checkpointed run")
-    ap.add_argument("--autotune", type=int, default=2, metavar="TRIAL_STEPS",
+    ap.add_argument("--autotune", type=int, default=0, metavar="TRIAL_STEPS",
                     help="autotune the launch configuration on the warmed-up state before "
-                         "the timed region (0: off)")
+                         "the timed region (0: off; the configuration comes from --tuned)")
+    ap.add_argument("--tuned", type=str, default="auto", metavar="JSON",
+                    help="launch configuration of the timed region, as written by "
+                         "--write-tuned (auto: profiles/tuned_million_block_warmup<W>.json "
+                         "when present, else the library defaults; none: the defaults)")
+    ap.add_argument("--write-tuned", type=str, default="", metavar="JSON",
+                    help="with --autotune: write the tuner's choice for this warm-up here")
     ap.add_argument("--scenes", type=int, default=256, help="ensemble size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (counter calibration)")
@@ -583,7 +621,10 @@ def main():
     if args.summarize_rocprof:
         print(summarize_rocprof(args.summarize_rocprof, args.steps,
                                 {"lambda": 4, "delta": 4, "xsph": 1, "vort_omega": 1,
-                                 "vort_apply": 1, "neighbors": 1, "reorder": 1}))
+                                 "vort_apply": 1, "neighbors": 1, "reorder": 1,
+                                 "tile_lambda": 4, "tile_delta": 4, "tile_xsph": 1,
+                                 "tile_vort_omega": 1, "tile_vort_apply": 1,
+                                 "tile_neighbors": 1, "tile_classify": 1}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -599,31 +640,45 @@ def main():
     dims = tuple(int(x) for x in args.dims.split(","))
     sc = scenes.million_block(dims=dims)
     n = sc.state.n
-    s = hip.HipSolver(sc.params, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap),
-                      brick_edge=args.brick_edge)
-    s.upload(sc.state.pos, sc.state.vel, 0.0)
     prof = not args.no_profile and not args.graph
     early = None
-    warm_left = args.warmup
     if prof and args.warmup >= 3:
-        # substeps 2-3 of the warm-up, before particles pile up (about 28
-        # neighbors each): the regime the algorithmic byte model describes
-        s.step(1)
-        s.profile(True)
-        s.step(2)
-        early_k = read_kernels(s)
-        s.profile(False)
-        pairs = s.stats()["pairs"]
-        early = roofline_record(early_k, n, mean_neighbors=pairs / n)
+        # substeps 2-3 from the start, before particles pile up (about 28
+        # neighbors each): the regime the algorithmic byte model describes,
+        # in a context of its own with the default launch configuration
+        se = hip.HipSolver(sc.params, [n])
+        se.upload(sc.state.pos, sc.state.vel, 0.0)
+        se.step(1)
+        se.profile(True)
+        se.step(2)
+        early_k = read_kernels(se)
+        se.profile(False)
+        st_e = se.stats()
+        se.close()
+        early = roofline_record(early_k, n, mean_neighbors=st_e["pairs"] / n)
         early["substeps"] = "2-3"
-        early["mean_neighbors"] = pairs / n
+        early["mean_neighbors"] = st_e["pairs"] / n
+        early["tiled_particles"] = st_e["tiled_particles"]
         early["kernels"] = early_k
-        warm_left -= 3
-    s.step(warm_left, graph=args.graph)
+    launch, launch_src = tuned_config(args)
+    s = hip.HipSolver(sc.params, [n], block_size=launch["block_size"],
+                      xcd_remap=bool(launch["xcd_remap"]), wave_mode=launch["wave_mode"],
+                      brick_edge=launch["brick_edge"], tile_gather=launch["tile_gather"])
+    s.upload(sc.state.pos, sc.state.vel, 0.0)
+    s.step(args.warmup, graph=args.graph)
     start_state = s.download() if not args.no_cpu else None
     # the tuner's trials start from this state and leave it in place (they
     # also size the tables for it); the timed region starts here too
     tuned = s.autotune(args.autotune) if args.autotune > 0 else None
+    if tuned:
+        launch = {k: tuned[k] for k in TUNED_KEYS}
+        launch_src = f"autotune({args.autotune}) on the state after {args.warmup} substeps"
+        if args.write_tuned:
+            with open(args.write_tuned, "w") as f:
+                json.dump({"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]}",
+                           "warmup": args.warmup, "launch": launch, "autotune": tuned,
+                           "command": " ".join(["python", "bench.py", *sys.argv[1:]])},
+                          f, indent=1)
     s.sync()
     if prof:
         s.profile(True)
@@ -639,22 +694,20 @@ def main():
         s.profile(False)
     st = s.stats()
     sizes = {"neighbor_pairs_last_step": st["pairs"], "list_words_last_step": st["words"],
-             "grid_cells_last_step": st["cells"],
+             "grid_cells_last_step": st["cells"], "tiled_particles_last_step": st["tiled_particles"],
              "capacity_growths_in_window": st["grow_events"] - grows_before}
     s.close()
 
     config = {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]} all flags, 4 iters",
               "dims": list(dims), "particles": n, "graph": bool(args.graph),
-              "block": tuned["block_size"] if tuned else args.block,
-              "xcd_remap": bool(tuned["xcd_remap"] if tuned else args.xcd_remap),
-              "wave_mode": tuned["wave_mode"] if tuned else 2,
-              "brick_edge": tuned["brick_edge"] if tuned else (args.brick_edge or 4)}
+              "block": launch["block_size"], "xcd_remap": bool(launch["xcd_remap"]),
+              "wave_mode": launch["wave_mode"], "brick_edge": launch["brick_edge"],
+              "tile_gather": launch["tile_gather"], "launch_from": launch_src}
     roofline = None
     if kernels:
         roofline = roofline_record(kernels, n, mean_neighbors=st["pairs"] / max(n, 1))
         run = {"dims": list(dims), "warmup": args.warmup, "steps": args.steps,
-               "config": {k: config[k] for k in ("block", "xcd_remap", "wave_mode", "brick_edge",
-                                                 "graph")}}
+               "config": {k: config[k] for k in RUN_KEYS}}
         traffic, l2, note = measured_traffic(roofline["kernel"], run)
         roofline["traffic"] = traffic
         roofline["l2_hit_rate"] = l2

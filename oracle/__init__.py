@@ -191,6 +191,14 @@ def step(params, pos, vel, nsteps=1, time=0.0, diag=False, which="oracle",
     return out
 
 
+def set_pow_mode(mode: int) -> int:
+    """s_corr's power for n outside {2, 3, 4}: 0 std::pow as the reference
+    (core.cpp:59-71), 1 the correctly rounded power (a sibling trajectory for
+    measuring how far a 1-ulp pow difference carries; not the reference).
+    Returns the previous mode."""
+    return int(_load(ORACLE_SO).oracle_set_pow_mode(int(mode)))
+
+
 def set_threads(t: int, which="oracle") -> int:
     lib = _load(ORACLE_SO if which == "oracle" else REF_SO)
     f = lib.oracle_set_threads if which == "oracle" else lib.ref_set_threads

@@ -102,6 +102,12 @@ T spiky_factor(T r, T h) {
   return coeff * diff * diff;
 }
 
+// 0: std::pow as the reference calls it; 1: the correctly rounded power (the
+// double-precision power rounded to T).  Mode 1 is not the reference: it
+// runs a sibling trajectory whose only difference is a pow within 1 ulp, the
+// yardstick for how far such a difference carries in a given scene.
+int g_pow_mode = 0;
+
 template <typename T>
 T pow_n(T ratio, int n) {
   // core/src/core.cpp:59-71
@@ -110,6 +116,13 @@ T pow_n(T ratio, int n) {
   if (n == 4) {
     const T r2 = ratio * ratio;
     return r2 * r2;
+  }
+  if (g_pow_mode == 1)
+    return static_cast<T>(std::pow(static_cast<double>(ratio), static_cast<double>(n)));
+  if (g_pow_mode == 2) {  // a product chain: within about n/2 ulp
+    T r = ratio;
+    for (int i = 1; i < n; ++i) r *= ratio;
+    return r;
   }
   return std::pow(ratio, static_cast<T>(n));
 }
@@ -645,6 +658,14 @@ int run_steps(const OracleParams* P, long long n, T* pos, T* vel, int nsteps,
 }  // namespace
 
 extern "C" {
+
+// s_corr's power for exponents outside {2, 3, 4}: 0 as the reference, 1 the
+// correctly rounded power (see g_pow_mode).  Returns the previous mode.
+int oracle_set_pow_mode(int mode) {
+  const int old = g_pow_mode;
+  g_pow_mode = mode;
+  return old;
+}
 
 // pos/vel are SoA blocks of 3*n (x block, y block, z block), updated in place.
 int oracle_step_f32(const OracleParams* P, long long n, float* pos, float* vel,

@@ -7,7 +7,7 @@ fixtures (no pickles): initial pos/vel, the parameters, and after `steps`
 steps the positions, velocities, time, and the last step's cells, sorted
 order, neighbor counts and lists, rho and lambda.
 
-    python tests/golden/make_golden.py        (needs oracle/_ref built)
+    python tests/golden/make_golden.py [NAME ...]   (needs oracle/_ref built)
 """
 from __future__ import annotations
 
@@ -48,6 +48,15 @@ def cases():
     out.append(("scorr_1000_2step", p, mid, zeros_m, 2))
     p = base.copy(enable_scorr=True, scorr_k=0.1, scorr_n=3, scorr_dq_coeff=0.0)
     out.append(("scorr_n3_dqdefault_1000_1step", p, mid, zeros_m, 1))
+    p = base.copy(enable_scorr=True, scorr_k=0.1, scorr_n=2, scorr_dq_coeff=0.2)
+    out.append(("scorr_n2_1000_2step", p, mid, zeros_m, 2))
+    # n outside {2, 3, 4}: the reference's std::pow branch (core.cpp:59-71);
+    # the device's powf may differ from the host libm's in the last bit, so
+    # these are compared to a stated tolerance, not bitwise ("powf_" prefix)
+    p = base.copy(enable_scorr=True, scorr_k=0.1, scorr_n=5, scorr_dq_coeff=0.2)
+    out.append(("powf_scorr_n5_1000_1step", p, mid, zeros_m, 1))
+    p = base.copy(enable_scorr=True, scorr_k=0.1, scorr_n=5, scorr_dq_coeff=0.2)
+    out.append(("powf_scorr_n5_1000_3step", p, mid, zeros_m, 3))
     p = base.copy(enable_xsph=True, visc_c=0.01)
     out.append(("xsph_1000_2step", p, mid, zeros_m, 2))
     p = base.copy(enable_vorticity=True, vort_epsilon=5e-4)
@@ -77,10 +86,13 @@ def cases():
     return out
 
 
-def main():
+def main(only=()):
+    """Writes every case, or only the cases named on the command line."""
     if not oracle.have_ref():
         raise SystemExit("oracle/_ref/libfluid_ref.so is not built (make -C oracle ref)")
     for name, p, pos, vel, steps in cases():
+        if only and name not in only:
+            continue
         r = oracle.step(p, pos, vel, nsteps=steps, diag=True, which="ref")
         np.savez_compressed(
             os.path.join(OUT, f"{name}.npz"),
@@ -93,4 +105,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))
