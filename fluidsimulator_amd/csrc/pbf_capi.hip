@@ -667,7 +667,7 @@ bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
   take(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   take(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   take(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
-  take(c.tile_list, static_cast<size_t>(fluidhip::kTileList) * n);
+  take(c.tile_list, static_cast<size_t>(fluidhip::kTileList / 8) * n);
   take(c.tile_cnt, n);
   if (!ok) return false;
   off = (off + 255) & ~static_cast<size_t>(255);
@@ -862,7 +862,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   ALLOC(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   ALLOC(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
-  ALLOC(c.tile_list, static_cast<size_t>(fluidhip::kTileList) * n);
+  ALLOC(c.tile_list, static_cast<size_t>(fluidhip::kTileList / 8) * n);
   ALLOC(c.tile_cnt, n);
 #undef ALLOC
   e = hipMemcpy(c.n_dev, &c.n, sizeof(int), hipMemcpyHostToDevice);
@@ -1211,7 +1211,7 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
     HIP_TRY(hipMemcpy(tcnt.data(), c.tile_cnt, sizeof(int) * n, hipMemcpyDeviceToHost),
             "read tile lists");
   }
-  // neighbor slots of tile slot s, in list order: entry r is an LDS record of
+  // neighbor slots of tile slot s, in list order: each entry is an LDS record of
   // the tile's halo, the halo cells' records laid out by tile_seg (first
   // record of each cell) and tile_src (its first slot)
   auto tile_neighbors = [&](size_t s, std::vector<int>& outs) {
@@ -1219,7 +1219,8 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
     const int* seg = tseg.data() + i * 217;
     const int* src = tsrc.data() + i * 216;
     for (int k = 0; k < tcnt[s]; ++k) {
-      const int r = tlist[static_cast<size_t>(k) * n + s];
+      // chunk k / 8 of particle s holds entries 8 (k / 8) .. 8 (k / 8) + 7
+      const int r = tlist[(static_cast<size_t>(k >> 3) * n + s) * 8 + (k & 7)];
       const int cc = static_cast<int>(std::upper_bound(seg, seg + 216, r) - seg) - 1;
       outs.push_back(src[cc] + (r - seg[cc]));
     }

@@ -26,7 +26,7 @@ struct TileArgs {
   const int4* tile_rec;    // per tile: {a, e, halo records, scene}, {first cell x, y, z, brick}
   const int* tile_seg;     // per tile: 217 LDS offsets of the halo cells
   const int* tile_src;     // per tile: 216 first slots of the halo cells
-  const uint16_t* tile_list;  // kTileList x n: neighbors' LDS records
+  const uint4* tile_list;  // kTileList / 8 x n chunks of 8 neighbors' LDS records
   const int* tile_cnt;        // n: neighbors of a tile particle
 };
 
@@ -124,7 +124,7 @@ struct Ctx {
   int4* tile_rec = nullptr;    // 2 tile_cap: tile records
   int* tile_seg = nullptr;     // tile_cap x 217: LDS offset of each halo cell
   int* tile_src = nullptr;     // tile_cap x 216: first slot of each halo cell
-  uint16_t* tile_list = nullptr;  // kTileList x n: neighbors' LDS records, entry-major
+  uint4* tile_list = nullptr;  // kTileList / 8 x n: chunks of 8 neighbors' LDS records (16 bit)
   int* tile_cnt = nullptr;        // n: neighbors of a tile particle
   long long tile_cap = 0;
 

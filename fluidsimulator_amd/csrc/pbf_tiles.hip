@@ -25,8 +25,9 @@
 //
 // In halo order the three cells (dx = -1, 0, 1) of one (dz, dy) row are
 // consecutive, so a particle's 27 cells are 9 contiguous LDS segments.  A
-// list entry is 16 bits (an LDS record) and the lists are entry-major
-// (entry k of every particle together), so the loads of one pass coalesce.
+// list entry is 16 bits (an LDS record); lists are chunks of 8 entries, chunk
+// q of every particle together, so the loads of one pass coalesce and one
+// 16-byte load serves 8 visits.
 //
 // Every tile kernel is persistent: a fixed grid loops over the tiles the
 // classify kernel found (a device count), so no launch depends on host-side
@@ -230,15 +231,23 @@ __device__ __forceinline__ void stage_halo(const TileLayout& L, int nrec, Fetch&
 }
 
 // Calls visit(r) for the LDS records r of a tile particle's cnt neighbors,
-// in list (= candidate) order; each entry is loaded one visit ahead.
+// in list (= candidate) order.  A list is chunks of 8 16-bit entries (one
+// uint4, chunk-major: chunk q of every particle together, so one load of a
+// pass coalesces); each chunk is loaded one chunk ahead, and the 8 visits of
+// a chunk are unrolled (a visit that no lane of the wavefront needs is
+// skipped as a whole).
 template <typename Visit>
-__device__ __forceinline__ void tile_walk(const uint16_t* __restrict__ list, int n, int s,
-                                          int cnt, int first, Visit&& visit) {
-  int r = first;
-  for (int k = 0; k < cnt; ++k) {
-    const int rn = k + 1 < cnt ? static_cast<int>(list[static_cast<size_t>(k + 1) * n + s]) : 0;
-    visit(r);
-    r = rn;
+__device__ __forceinline__ void tile_walk(const uint4* __restrict__ list, int n, int s, int cnt,
+                                          uint4 first, Visit&& visit) {
+  uint4 q = first;
+  for (int k = 0; k < cnt; k += 8) {
+    const uint4 nq = k + 8 < cnt ? list[static_cast<size_t>((k >> 3) + 1) * n + s]
+                                 : make_uint4(0u, 0u, 0u, 0u);
+    const unsigned w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (k + e < cnt) visit(static_cast<int>((w[e >> 1] >> (16 * (e & 1))) & 0xffffu));
+    q = nq;
   }
 }
 
@@ -261,13 +270,14 @@ __device__ __forceinline__ int tiles_of(const int* tile_count, int tile_cap) {
   return t < tile_cap ? t : tile_cap;
 }
 
-// A tile particle's neighbor count and first list entry.
+// A tile particle's neighbor count and first list chunk.
 struct Nbrs {
-  int cnt, first;
+  int cnt;
+  uint4 first;  // the list's first chunk
 };
 
 __device__ __forceinline__ Nbrs load_nbrs(const TileArgs& ta, int n, int s) {
-  return Nbrs{ta.tile_cnt[s], static_cast<int>(ta.tile_list[s])};
+  return Nbrs{ta.tile_cnt[s], ta.tile_list[s]};
 }
 
 // The tile loop of every gather kernel.  Each workgroup takes tiles in turn
@@ -276,7 +286,7 @@ __device__ __forceinline__ Nbrs load_nbrs(const TileArgs& ta, int n, int s) {
 //   1. loads the tile record and the halo layout (one round of loads),
 //   2. issues the halo gather (stage_halo's fetch and store) together with the
 //      loads of its first particle: load(s), the neighbor count and the first
-//      list entry (a second round),
+//      list chunk (a second round),
 //   3. runs body(s, tile, nbrs, x) for each particle s of the tile.
 template <typename X, typename Load, typename Fetch, typename Store, typename Body>
 __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, TileLayout& L,
@@ -330,7 +340,7 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
   const int nt = tiles_of(ta.tile_count, ta.tile_cap);
   const int per = (nt + 7) >> 3;
   const int qend = c.xcd_remap ? per << 3 : nt;
-  uint16_t* list = const_cast<uint16_t*>(ta.tile_list);
+  uint4* list = const_cast<uint4*>(ta.tile_list);
   int* cnt_out = const_cast<int*>(ta.tile_cnt);
   for (int q = blockIdx.x; q < qend; q += gridDim.x) {
     const int i = c.xcd_remap ? (q & 7) * per + (q >> 3) : q;
@@ -352,15 +362,34 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
       const int own = ((lz + 1) * kHalo + ly + 1) * kHalo + lx + 1;
       const int self = L.seg[own] + (s - L.src[own]);
       const float3 me = make_float3(X[self], Y[self], Z[self]);
+      // entries enter a 128-bit shift register w3:w2:w1:w0 at the top, 16
+      // bits at a time, so after 8 of them entry 0 is w0's low half; a chunk
+      // goes out every 8 entries
       int k = 0;
+      unsigned w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u;
       for (int row = 0; row < 9; ++row) {
         const int hc = ((lz + row / 3) * kHalo + ly + row % 3) * kHalo + lx;
         const int lo = L.seg[hc], hi = L.seg[hc + 3];
         for (int r = lo; r < hi; ++r) {
           if (r == self || !near0(me, make_float3(X[r], Y[r], Z[r]), c)) continue;
-          if (k < kTileList) list[static_cast<size_t>(k) * c.n + s] = static_cast<uint16_t>(r);
+          w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+          w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+          w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+          w3 = (w3 >> 16) | (static_cast<unsigned>(r) << 16);
+          if ((k & 7) == 7 && k < kTileList)
+            list[static_cast<size_t>(k >> 3) * c.n + s] = make_uint4(w0, w1, w2, w3);
           ++k;
         }
+      }
+      if ((k & 7) != 0 && k < kTileList) {
+        // the last m < 8 entries sit in the top m halves: shift them down
+        for (int m = k & 7; m < 8; ++m) {
+          w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+          w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+          w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+          w3 >>= 16;
+        }
+        list[static_cast<size_t>(k >> 3) * c.n + s] = make_uint4(w0, w1, w2, w3);
       }
       cnt_out[s] = k;
       ncount[s] = k;
