@@ -334,28 +334,35 @@ def bench_ensemble(args) -> dict:
     pos = np.concatenate([st.pos for st in ens.states], axis=1)
     vel = np.concatenate([st.vel for st in ens.states], axis=1)
 
-    def run_batched(graph):
+    tg = {"auto": 0, "on": 1, "off": 2}[args.tiles]
+
+    def run_batched(graph, prof=False):
         s = hip.HipSolver(ens.params, counts, block_size=args.block,
-                          xcd_remap=bool(args.xcd_remap))
+                          xcd_remap=bool(args.xcd_remap), tile_gather=tg)
         s.upload(pos, vel, 0.0)
         s.step(args.warmup, graph=graph)
         s.sync()
+        if prof:
+            s.profile(True)
         t0 = time.perf_counter()
         s.step(args.steps, graph=graph)
         s.sync()
         dt = time.perf_counter() - t0
+        kern = read_kernels(s) if prof else None
+        st = s.stats()
         out = s.download()
         s.close()
-        return dt, out
+        return dt, out, kern, st
 
-    t_graph, out_graph = run_batched(True)
-    t_eager, out_eager = run_batched(False)
+    t_graph, out_graph, _, st_graph = run_batched(True)
+    t_eager, out_eager, _, _ = run_batched(False)
+    _, _, kern, _ = run_batched(False, prof=not args.no_profile)
 
     # one context per scene (created before the timed region, like the batch),
     # looped eagerly and then with a captured graph per scene
     solos = [hip.HipSolver(p, [st.n], cell_capacity=1 << 17,
                            neighbor_capacity=64 * st.n + 4096, block_size=args.block,
-                           xcd_remap=bool(args.xcd_remap))
+                           xcd_remap=bool(args.xcd_remap), tile_gather=tg)
              for p, st in zip(ens.params, ens.states)]
     off = np.concatenate([[0], np.cumsum(counts)])
     same = True
@@ -406,6 +413,9 @@ def bench_ensemble(args) -> dict:
         "speedup_vs_per_scene_loop": t_loop / t_graph,
         "graph_vs_eager": t_eager / t_graph,
         "bitwise_equal_to_per_scene": same,
+        "tiles": args.tiles,
+        "tiled_particles_last_step": st_graph["tiled_particles"],
+        "kernels_eager": kern,
     }
 
 
@@ -420,7 +430,8 @@ def adjoint_run(args, steps, coefficients, every) -> dict:
     sc = scenes.rollout_scene(steps=steps, coefficients=coefficients)
     tg = scenes.rollout_scene(seed=6, steps=steps, coefficients=coefficients)
     p, n = sc.params, sc.state.n
-    s = hip.HipSolver(p, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap))
+    s = hip.HipSolver(p, [n], block_size=args.block, xcd_remap=bool(args.xcd_remap),
+                      tile_gather={"auto": 0, "on": 1, "off": 2}[args.tiles])
     s.upload(tg.state.pos, tg.state.vel)
     s.step(steps)
     target = s.download().pos.astype(np.float64)
@@ -596,6 +607,9 @@ def main():
     ap.add_argument("--write-tuned", type=str, default="", metavar="JSON",
                     help="with --autotune: write the tuner's choice for this warm-up here")
     ap.add_argument("--scenes", type=int, default=256, help="ensemble size")
+    ap.add_argument("--tiles", choices=["auto", "on", "off"], default="auto",
+                    help="ensemble and adjoint workloads: the LDS-tiled passes (auto: the "
+                         "library default, on with brick edge 4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (counter calibration)")
     ap.add_argument("--summarize-rocprof", type=str, default="", metavar="DIR",

@@ -265,6 +265,7 @@ void free_all(Ctx& c) {
   dfree(c.grid);
   dfree(c.in_tile);
   dfree(c.tile_count);
+  dfree(c.tile_cand);
   dfree(c.tile_rec);
   dfree(c.tile_seg);
   dfree(c.tile_src);
@@ -595,6 +596,18 @@ int noalloc_scratch(int dev, unsigned char** out) {
   return FLUID_HIP_OK;
 }
 
+// The context's own streams and events (the main stream only when it is not
+// the caller's).
+void destroy_streams(Ctx& c) {
+  if (c.ev_fork) hipEventDestroy(c.ev_fork);
+  if (c.ev_join) hipEventDestroy(c.ev_join);
+  if (c.side) hipStreamDestroy(c.side);
+  if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
+  c.ev_fork = c.ev_join = nullptr;
+  c.side = nullptr;
+  c.stream = nullptr;
+}
+
 // Lays the buffers of an n-particle, one-scene context out in [base, base +
 // bytes): the per-particle and fixed arrays first, then the rest split
 // between the neighbor lists (5/8) and the cell table (3/8).  Returns false
@@ -661,9 +674,10 @@ bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
   take(c.planes, 1);
   take(c.flags, 1);
   take(c.steps_started, 1);
-  c.tile_cap = static_cast<long long>(n / 16 + 256);
+  c.tile_cap = static_cast<long long>(n / 32 + 256);
   take(c.in_tile, n);
-  take(c.tile_count, 1);
+  take(c.tile_count, 2);
+  take(c.tile_cand, static_cast<size_t>(c.tile_cap));
   take(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   take(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   take(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
@@ -782,7 +796,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
 
   auto bail = [&](int code) {
     free_all(c);
-    if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
+    destroy_streams(c);
     delete h;
     return code;
   };
@@ -791,6 +805,12 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
   c.own_stream = true;
+  // the side stream of the LDS-tile kernels and its fork/join events
+  e = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+  e = hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
 
   const size_t n = static_cast<size_t>(c.n);
   const bool need_omega = c.vort;
@@ -855,10 +875,12 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.flags, 1);
   ALLOC(c.steps_started, 1);
   ALLOC(c.stage, 6 * n);
-  // brick tiles: beyond this capacity bricks take the per-lane passes
-  c.tile_cap = static_cast<long long>(n / 16 + 256);
+  // brick tiles (at least 32 particles each): beyond this capacity bricks
+  // take the per-lane passes
+  c.tile_cap = static_cast<long long>(n / 32 + 256);
   ALLOC(c.in_tile, n);
-  ALLOC(c.tile_count, 1);
+  ALLOC(c.tile_count, 2);
+  ALLOC(c.tile_cand, static_cast<size_t>(c.tile_cap));
   ALLOC(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   ALLOC(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   ALLOC(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
@@ -904,8 +926,9 @@ int fluid_hip_destroy(fluid_hip_ctx* ctx) {
   if (!ctx) return FLUID_HIP_OK;
   hipSetDevice(ctx->c.device);
   if (ctx->c.stream) hipStreamSynchronize(ctx->c.stream);
+  if (ctx->c.side) hipStreamSynchronize(ctx->c.side);
   free_all(ctx->c);
-  if (ctx->c.own_stream && ctx->c.stream) hipStreamDestroy(ctx->c.stream);
+  destroy_streams(ctx->c);
   delete ctx;
   return FLUID_HIP_OK;
 }

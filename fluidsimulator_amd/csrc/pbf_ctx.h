@@ -46,6 +46,12 @@ struct Prof {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // a second stream of the context (default priority) on which each pass's
+  // LDS-tile kernel runs beside the per-lane kernel, forked from and joined
+  // back into `stream` by the two events; null: the tile kernels run on
+  // `stream` (contexts built for one call, e.g. the allocation-free entry)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool own_stream = false;
 
   // problem shape
@@ -120,7 +126,8 @@ struct Ctx {
 
   // brick tiles (sparse bricks whose passes run through LDS)
   int* in_tile = nullptr;      // per slot: 1 + its tile's index this substep, 0 outside tiles
-  int* tile_count = nullptr;   // tiles found this substep
+  int* tile_count = nullptr;   // [0] tiles found this substep, [1] candidate bricks
+  int* tile_cand = nullptr;    // tile_cap: bricks of at least 32 particles (classify input)
   int4* tile_rec = nullptr;    // 2 tile_cap: tile records
   int* tile_seg = nullptr;     // tile_cap x 217: LDS offset of each halo cell
   int* tile_src = nullptr;     // tile_cap x 216: first slot of each halo cell
@@ -257,11 +264,14 @@ hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k);
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
                                   const float4* p0);
 hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
-                               int write_rho);
+                               int write_rho, hipStream_t st);
 hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,
-                              float4* Pn, bool last, bool rest, float4* vout);
-hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest);
-hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta);
-hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest);
+                              float4* Pn, bool last, bool rest, float4* vout, hipStream_t st);
+hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
+                             hipStream_t st);
+hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta,
+                                   hipStream_t st);
+hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
+                                   hipStream_t st);
 
 }  // namespace fluidhip

@@ -351,7 +351,10 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
   }
   if (threadIdx.x == 0 && !was_fatal) {
     *nbr_total = 0ull;
-    if (tile_count) *tile_count = 0;
+    if (tile_count) {
+      tile_count[0] = 0;  // tiles
+      tile_count[1] = 0;  // candidate bricks
+    }
     if (carry > cell_cap) {
       atomicOr(flags, kFlagCells);
       *total_cells = 0;
@@ -2404,6 +2407,26 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
   }));
 
+  // Each pass's tile kernel runs on the side stream beside the per-lane
+  // kernel and its run fill on the main stream: they write disjoint slots
+  // (tile bricks hold no run of coincident particles, and the lane map leaves
+  // tile slots out) and read only what earlier passes finished.  fork() makes
+  // the side stream wait for the main stream's work so far, join() the main
+  // stream for the side stream's.  Both are edges of a captured graph too.
+  const bool forked = tiles && c.side != nullptr;
+  hipStream_t tst = forked ? c.side : st;
+  hipError_t fj = hipSuccess;
+  auto fork = [&] {
+    if (!forked || fj != hipSuccess) return;
+    fj = hipEventRecord(c.ev_fork, st);
+    if (fj == hipSuccess) fj = hipStreamWaitEvent(c.side, c.ev_fork, 0);
+  };
+  auto join = [&] {
+    if (!forked || fj != hipSuccess) return;
+    fj = hipEventRecord(c.ev_join, c.side);
+    if (fj == hipSuccess) fj = hipStreamWaitEvent(st, c.ev_join, 0);
+  };
+
   // velocity buffer that the commit writes: XSPH needs a second buffer to
   // stay Jacobi, so the commit goes to velB and XSPH back to velA.
   const NbrView nv{c.noff, c.nwords, c.nbr, c.ncount};
@@ -2418,12 +2441,20 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     unsigned* pm = rec ? rec->M[it] : nullptr;
     unsigned* rm = rec ? rec->RM : nullptr;
     FH_TRY(L("lambda", [&] {
-      if (tiles) (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0);
+      if (tiles) {
+        fork();
+        (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0, tst);
+      }
       k_lambda<<<nl, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
       fill(P, nullptr);
+      if (tiles) join();
     }));
+    FH_TRY(fj);
     FH_TRY(L("delta", [&] {
-      if (tiles) (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1);
+      if (tiles) {
+        fork();
+        (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1, tst);
+      }
       if (!last)
         k_delta<true, false, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
                                                        c.planes, c.posB, c.posA, v1, c.rho,
@@ -2442,7 +2473,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         fill(c.posA, c.posB);  // each slot keeps its own id
         fill(v1, nullptr);
       }
+      if (tiles) join();
     }));
+    FH_TRY(fj);
   }
   if (I == 0) {
     FH_TRY(L("delta", [&] {
@@ -2461,7 +2494,10 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (c.xsph) {
     const bool rest = c.restitution_any && !c.vort;
     FH_TRY(L("xsph", [&] {
-      if (tiles) (void)enqueue_tile_xsph(c, k, ta, rest);
+      if (tiles) {
+        fork();
+        (void)enqueue_tile_xsph(c, k, ta, rest, tst);
+      }
       if (rest)
         k_xsph<true><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                         c.planes, rec ? rec->RM : nullptr, c.flags);
@@ -2469,16 +2505,26 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         k_xsph<false><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
                                          c.planes, nullptr, c.flags);
       fill(c.velA, nullptr);
+      if (tiles) join();
     }));
+    FH_TRY(fj);
   }
   if (c.vort) {
     FH_TRY(L("vorticity_omega", [&] {
-      if (tiles) (void)enqueue_tile_vort_omega(c, k, ta);
+      if (tiles) {
+        fork();
+        (void)enqueue_tile_vort_omega(c, k, ta, tst);
+      }
       k_vort_omega<<<nl, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
       fill(c.omega, nullptr);
+      if (tiles) join();
     }));
+    FH_TRY(fj);
     FH_TRY(L("vorticity_apply", [&] {
-      if (tiles) (void)enqueue_tile_vort_apply(c, k, ta, c.restitution_any);
+      if (tiles) {
+        fork();
+        (void)enqueue_tile_vort_apply(c, k, ta, c.restitution_any, tst);
+      }
       if (c.restitution_any)
         k_vort_apply<true><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                               c.planes, c.flags);
@@ -2486,7 +2532,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
         k_vort_apply<false><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,
                                                c.planes, c.flags);
       fill(c.velA, nullptr);
+      if (tiles) join();
     }));
+    FH_TRY(fj);
   }
 #undef FH_TRY
   return hipSuccess;

@@ -3,9 +3,12 @@
 // The cell table orders cells in 4x4x4-cell bricks (brick_shift 2), so a
 // brick's particles are one contiguous slot range and the 6x6x6 cells around
 // it (the brick and a one-cell halo) hold every candidate of those particles.
-// A non-empty brick is a tile when its whole halo holds at most kTileMaxHalo
-// particles (and, found by the neighbor kernel, none of its particles has more
-// than kTileList neighbors).  For a tile:
+// A brick of at least kTileMinParticles particles is a tile when its whole
+// halo holds at most kTileMaxHalo particles, each of its non-empty cells'
+// 27-cell neighborhoods fewer than
+// kTileMaxCand, no two of its particles coincide (the per-lane passes compute
+// a run of coincident particles once) and, found by the neighbor kernel, none
+// of its particles has more than kTileList neighbors.  For a tile:
 //
 //   * classify (once per substep): records the tile, the LDS offset of each
 //     halo cell (the cells' particle counts, scanned in halo order z, y, x)
@@ -48,9 +51,22 @@ constexpr int kHaloCells = 216;
 constexpr int kThreads = 256;      // workgroup of the classify kernel
 constexpr int kPass = 512;         // workgroup of the neighbor and gather kernels
 constexpr int kPerThread = kTileMaxHalo / kPass;  // halo records staged per thread
+// Bricks with a cell of this many candidates (27-cell particles) or more take
+// the per-lane passes: the tile neighbor build tests every candidate.
+constexpr int kTileMaxCand = 512;
+// Bricks with fewer particles take the per-lane passes: a tile's fixed cost
+// (layout, halo staging, a 512-thread workgroup) is not worth it for a few
+// particles of sparse spray.
+constexpr int kTileMinParticles = 32;
 
 __device__ __forceinline__ bool tile_fatal(const unsigned* flags) {
   return (*flags & kFlagFatal) != 0u;
+}
+
+// Bitwise equal positions.
+__device__ __forceinline__ bool coincide(const float3& a, const float3& b) {
+  return ((__float_as_uint(a.x) ^ __float_as_uint(b.x)) | (__float_as_uint(a.y) ^ __float_as_uint(b.y)) |
+          (__float_as_uint(a.z) ^ __float_as_uint(b.z))) == 0u;
 }
 
 // Scene of the brick whose first cell has key `k0` (scenes own whole bricks).
@@ -105,28 +121,63 @@ __device__ __forceinline__ int tile_excl_scan(int v, int* lds, int* total) {
 
 // ------------------------------------------------------------ classify
 
-// Finds this substep's tiles among the bricks of the cell table.  in_tile[s]
-// is written for every slot: 1 + the tile's index when its brick is a tile,
-// else 0.
+// Bricks of at least kTileMinParticles particles, appended to tile_cand
+// (tile_count[1] of them; one atomic per wavefront); the slots of smaller
+// bricks get in_tile = 0 here.  A brick of >= kTileMinParticles holds at
+// least 1/kTileMinParticles of the particles, so tile_cap entries suffice.
+__global__ __launch_bounds__(kThreads) void k_tile_bricks(
+    const int* __restrict__ start, const int* __restrict__ total_cells, int* __restrict__ in_tile,
+    int* __restrict__ tile_count, int* __restrict__ tile_cand, int tile_cap,
+    const unsigned* __restrict__ flags) {
+  if (tile_fatal(flags)) return;
+  const int nbricks = *total_cells >> 6;
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts iterate together (the ballot needs every lane)
+  for (int b0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; b0 < nbricks;
+       b0 += gridDim.x * blockDim.x) {
+    const int b = b0 + lane;
+    bool big = false;
+    if (b < nbricks) {
+      const int a = start[b << 6];
+      const int e = start[(b << 6) + 64];
+      big = e - a >= kTileMinParticles;
+      if (!big)
+        for (int s = a; s < e; ++s) in_tile[s] = 0;
+    }
+    const unsigned long long m = __ballot(big);
+    if (m == 0ull) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(tile_count + 1, __popcll(m));
+    base = __shfl(base, 0, 64);
+    const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (big && at < tile_cap) tile_cand[at] = b;
+  }
+}
+
+// Examines the candidate bricks (one workgroup at a time, persistent grid):
+// a tile gets a record, its halo layout and in_tile[s] = 1 + its index for
+// its slots; the other candidates' slots get in_tile = 0.
 __global__ __launch_bounds__(kThreads) void k_tile_classify(
     StepConsts c, const GridDesc* __restrict__ grid, const int* __restrict__ start,
-    const int* __restrict__ total_cells, int* __restrict__ in_tile, int* __restrict__ tile_count,
-    int4* __restrict__ tile_rec, int* __restrict__ tile_seg, int* __restrict__ tile_src,
-    int tile_cap, const unsigned* __restrict__ flags) {
+    const float4* __restrict__ posB, const int* __restrict__ tile_cand,
+    int* __restrict__ in_tile, int* __restrict__ tile_count, int4* __restrict__ tile_rec,
+    int* __restrict__ tile_seg, int* __restrict__ tile_src, int tile_cap,
+    const unsigned* __restrict__ flags) {
+  __shared__ int hcnt[kHaloCells];
   __shared__ int hsrc[kHaloCells];
   __shared__ int scan_lds[kThreads / 64 + 1];
   __shared__ int idx_sh;
   if (tile_fatal(flags)) return;
-  const int nbricks = *total_cells >> 6;
-  for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+  const int ncand = min(tile_count[1], tile_cap);
+  const int t = threadIdx.x;
+  for (int j = blockIdx.x; j < ncand; j += gridDim.x) {
+    const int b = tile_cand[j];
     const int a = start[b << 6];
     const int e = start[(b << 6) + 64];
-    if (a == e) continue;  // empty brick (uniform)
     const int sc = scene_of_key(b << 6, grid, c.n_scenes);
     const GridDesc g = grid[sc];
     int ox, oy, oz;
     brick_origin(b, g, ox, oy, oz);
-    const int t = threadIdx.x;
     int cnt = 0, src = 0;
     if (t < kHaloCells) {
       const int rx = ox - 1 + t % kHalo, ry = oy - 1 + (t / kHalo) % kHalo,
@@ -137,12 +188,31 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
         src = start[k];
         cnt = start[k + 1] - src;
       }
+      hcnt[t] = cnt;
       hsrc[t] = src;
     }
     __syncthreads();
     int total;
     const int off = tile_excl_scan(cnt, scan_lds, &total);
-    const bool ok = total <= kTileMaxHalo;
+    // a cell whose 27-cell neighborhood holds kTileMaxCand candidates or more
+    bool bad = false;
+    if (t < 64) {
+      // the brick's cell (x, y, z) = (t & 3, (t >> 2) & 3, t >> 4)
+      const int hx = (t & 3) + 1, hy = ((t >> 2) & 3) + 1, hz = (t >> 4) + 1;
+      if (hcnt[(hz * kHalo + hy) * kHalo + hx] > 0) {
+        int cand = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx)
+              cand += hcnt[((hz + dz) * kHalo + hy + dy) * kHalo + hx + dx];
+        bad = cand >= kTileMaxCand;
+      }
+    }
+    // coincident particles (consecutive slots at one position): the per-lane
+    // passes compute a run of them once
+    for (int s = a + 1 + t; s < e && !bad; s += kThreads)
+      bad = coincide(ld3(posB, s), ld3(posB, s - 1));
+    const bool ok = !__syncthreads_or(bad) && total <= kTileMaxHalo;
     if (ok && t == 0) idx_sh = atomicAdd(tile_count, 1);
     __syncthreads();
     const bool tile = ok && idx_sh < tile_cap;
@@ -302,8 +372,8 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
     if (T.dead) continue;
     load_layout(L, ta.tile_seg, ta.tile_src, i);
     int s = T.a + static_cast<int>(threadIdx.x);
-    // a tile holds at least one particle: lanes past its end load its last
-    // particle's data (in bounds) and do not use it
+    // a tile holds at least kTileMinParticles particles: lanes past its end
+    // load its last particle's data (in bounds) and do not use it
     const int s0 = s < T.e ? s : T.e - 1;
     Nbrs nb = load_nbrs(ta, n, s0);
     X x = load(s0);
@@ -700,8 +770,11 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_apply(
 // ------------------------------------------------------------ launching
 
 hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k) {
+  k_tile_bricks<<<1024, kThreads, 0, c.stream>>>(c.start, c.total_cells, c.in_tile, c.tile_count,
+                                                c.tile_cand, static_cast<int>(c.tile_cap),
+                                                c.flags);
   k_tile_classify<<<kTileGrid, kThreads, 0, c.stream>>>(
-      k, c.grid, c.start, c.total_cells, c.in_tile, c.tile_count, c.tile_rec, c.tile_seg,
+      k, c.grid, c.start, c.posB, c.tile_cand, c.in_tile, c.tile_count, c.tile_rec, c.tile_seg,
       c.tile_src, static_cast<int>(c.tile_cap), c.flags);
   return hipGetLastError();
 }
@@ -714,48 +787,51 @@ hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& t
 }
 
 hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
-                               int write_rho) {
-  k_tile_lambda<<<kTileGrid, kPass, 0, c.stream>>>(k, ta, P, c.rho, write_rho,
+                               int write_rho, hipStream_t st) {
+  k_tile_lambda<<<kTileGrid, kPass, 0, st>>>(k, ta, P, c.rho, write_rho,
                                                      c.flags);
   return hipGetLastError();
 }
 
 hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,
-                              float4* Pn, bool last, bool rest, float4* vout) {
+                              float4* Pn, bool last, bool rest, float4* vout, hipStream_t st) {
   if (!last)
-    k_tile_delta<false, false><<<kTileGrid, kPass, 0, c.stream>>>(
+    k_tile_delta<false, false><<<kTileGrid, kPass, 0, st>>>(
         k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
   else if (rest)
-    k_tile_delta<true, true><<<kTileGrid, kPass, 0, c.stream>>>(
+    k_tile_delta<true, true><<<kTileGrid, kPass, 0, st>>>(
         k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
   else
-    k_tile_delta<true, false><<<kTileGrid, kPass, 0, c.stream>>>(
+    k_tile_delta<true, false><<<kTileGrid, kPass, 0, st>>>(
         k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
   return hipGetLastError();
 }
 
-hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest) {
+hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
+                             hipStream_t st) {
   if (rest)
-    k_tile_xsph<true><<<kTileGrid, kPass, 0, c.stream>>>(k, ta, c.posA, c.velB, c.velA,
+    k_tile_xsph<true><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA,
                                                            c.planes, c.flags);
   else
-    k_tile_xsph<false><<<kTileGrid, kPass, 0, c.stream>>>(k, ta, c.posA, c.velB, c.velA,
+    k_tile_xsph<false><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA,
                                                             c.planes, c.flags);
   return hipGetLastError();
 }
 
-hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta) {
-  k_tile_vort_omega<<<kTileGrid, kPass, 0, c.stream>>>(k, ta, c.posA, c.velA, c.omega,
+hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta,
+                                   hipStream_t st) {
+  k_tile_vort_omega<<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velA, c.omega,
                                                          c.flags);
   return hipGetLastError();
 }
 
-hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest) {
+hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
+                                   hipStream_t st) {
   if (rest)
-    k_tile_vort_apply<true><<<kTileGrid, kPass, 0, c.stream>>>(k, ta, c.posA, c.omega, c.velA,
+    k_tile_vort_apply<true><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.omega, c.velA,
                                                                  c.planes, c.flags);
   else
-    k_tile_vort_apply<false><<<kTileGrid, kPass, 0, c.stream>>>(
+    k_tile_vort_apply<false><<<kTileGrid, kPass, 0, st>>>(
         k, ta, c.posA, c.omega, c.velA, c.planes, c.flags);
   return hipGetLastError();
 }
