@@ -315,25 +315,39 @@ __global__ __launch_bounds__(256) void k_adj_lambda(
 }
 
 // g-bar of each scene += dt^2 * sum of p0-bar over its slots, summed in a
-// fixed order in double.
-__global__ __launch_bounds__(256) void k_adj_gravity(StepConsts c, const int* __restrict__ seg,
-                                                     const float4* __restrict__ PB,
-                                                     double* __restrict__ gacc) {
-  __shared__ double lds[3][256];
+// fixed order in double: one workgroup of 1024 threads per scene, each thread
+// with four accumulators over interleaved slots (independent chains, so the
+// loads overlap), folded in a fixed order and then across the workgroup.
+__global__ __launch_bounds__(1024) void k_adj_gravity(StepConsts c, const int* __restrict__ seg,
+                                                      const float4* __restrict__ PB,
+                                                      double* __restrict__ gacc) {
+  constexpr int kT = 1024;
+  __shared__ double lds[3][kT];
   const int sc = blockIdx.x;
   const int a = seg[sc], b = seg[sc + 1];
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int i = a + static_cast<int>(threadIdx.x); i < b; i += 256) {
-    const float4 v = PB[i];
-    sx += v.x;
-    sy += v.y;
-    sz += v.z;
+  double sx[4] = {0.0, 0.0, 0.0, 0.0}, sy[4] = {0.0, 0.0, 0.0, 0.0},
+         sz[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = a + static_cast<int>(threadIdx.x);
+  for (; i + 3 * kT < b; i += 4 * kT) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 v = PB[i + u * kT];
+      sx[u] += v.x;
+      sy[u] += v.y;
+      sz[u] += v.z;
+    }
   }
-  lds[0][threadIdx.x] = sx;
-  lds[1][threadIdx.x] = sy;
-  lds[2][threadIdx.x] = sz;
+  for (int u = 0; i < b; i += kT, ++u) {
+    const float4 v = PB[i];
+    sx[u] += v.x;
+    sy[u] += v.y;
+    sz[u] += v.z;
+  }
+  lds[0][threadIdx.x] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
+  lds[1][threadIdx.x] = (sy[0] + sy[1]) + (sy[2] + sy[3]);
+  lds[2][threadIdx.x] = (sz[0] + sz[1]) + (sz[2] + sz[3]);
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = kT / 2; w > 0; w >>= 1) {
     if (static_cast<int>(threadIdx.x) < w) {
       lds[0][threadIdx.x] += lds[0][threadIdx.x + w];
       lds[1][threadIdx.x] += lds[1][threadIdx.x + w];
@@ -419,7 +433,7 @@ hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b
     k_adj_lambda<<<nb, 256, 0, st>>>(k, nv, P, b.LB, b.RB, b.PB, Mprev, b.W, t.X, b.XB, b.GX,
                                      b.GV, c.seg, c.planes);
   }
-  k_adj_gravity<<<c.n_scenes, 256, 0, st>>>(k, c.seg, b.PB, b.gacc);
+  k_adj_gravity<<<c.n_scenes, 1024, 0, st>>>(k, c.seg, b.PB, b.gacc);
   return hipGetLastError();
 }
 

@@ -666,7 +666,7 @@ bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
   take(c.n_dev, 1);
   take(c.grid, 1);
   take(c.total_cells, 1);
-  take(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
+  take(c.scan_partial, 2 * static_cast<size_t>(c.scan_blocks) + 2);
   take(c.nbr_total, 1);
   take(c.need_cells, 1);
   take(c.pair_total, 1);
@@ -866,7 +866,7 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.n_dev, 1);
   ALLOC(c.grid, static_cast<size_t>(n_scenes));
   ALLOC(c.total_cells, 1);
-  ALLOC(c.scan_partial, static_cast<size_t>(c.scan_blocks) + 1);
+  ALLOC(c.scan_partial, 2 * static_cast<size_t>(c.scan_blocks) + 2);
   ALLOC(c.nbr_total, 1);
   ALLOC(c.need_cells, 1);
   ALLOC(c.pair_total, 1);

@@ -16,6 +16,7 @@ namespace fluidhip {
 // Brick tiles of the LDS-tiled gather passes (pbf_tiles.hip).
 constexpr int kTileMaxHalo = 2048;  // particles of a tile's 6^3-cell halo held in LDS
 constexpr int kTileList = 128;      // neighbors of a tile particle, at most (else per-lane)
+constexpr int kForkMinParticles = 1 << 18;  // tile kernels on the side stream from this size
 constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
 
 // What the tile kernels read besides their pass's arrays.
@@ -102,7 +103,7 @@ struct Ctx {
   int* tile_bbox = nullptr;  // 6 per particle tile (workgroup)
   GridDesc* grid = nullptr;
   int* total_cells = nullptr;
-  int* scan_partial = nullptr;  // scan_blocks + 1
+  int* scan_partial = nullptr;  // 2 scan_blocks + 2: range sums of one or two scanned arrays
   uint64_t* slot_cell = nullptr;  // packed cell of pred0 per sorted slot
   int* slot_key = nullptr;  // table key of each sorted slot's cell
   // runs of identical consecutive slots (one lane computes each run)

@@ -431,7 +431,11 @@ __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
 
 // ---- exclusive scan over a device-sized array (three launches, fixed grid)
 
+// Exclusive scans of one array, or of two arrays of one length at once (in2
+// and out2 non-null), in two launches: k_scan_reduce writes each workgroup's
+// range sums to partial ([0, G) for in, [G, 2G) for in2), k_scan_down scans.
 __global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ in,
+                                                     const int* __restrict__ in2,
                                                      const int* __restrict__ d_len,
                                                      int* __restrict__ partial,
                                                      const unsigned* __restrict__ flags) {
@@ -442,31 +446,45 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ in,
   const int chunk = (len + G - 1) / G;
   const int b0 = blockIdx.x * chunk;
   const int b1 = min(len, b0 + chunk);
-  int acc = 0;
-  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) acc += in[i];
+  int acc = 0, acc2 = 0;
+  for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    acc += in[i];
+    if (in2) acc2 += in2[i];
+  }
   int total;
   block_excl_scan(acc, lds, &total);
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
+  if (in2) {
+    block_excl_scan(acc2, lds, &total);
+    if (threadIdx.x == 0) partial[G + blockIdx.x] = total;
+  }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_partials(int* __restrict__ partial, int G,
-                                                        const unsigned* __restrict__ flags) {
-  __shared__ int lds[17];
-  if (fatal(flags)) return;
-  const int v = (static_cast<int>(threadIdx.x) < G) ? partial[threadIdx.x] : 0;
-  int total;
-  const int e = block_excl_scan(v, lds, &total);
-  if (static_cast<int>(threadIdx.x) < G) partial[threadIdx.x] = e;
-  if (threadIdx.x == 0) partial[G] = total;
+// Sum of the partials of the workgroups before this one (*pre) and of all
+// of them (*grand).
+__device__ __forceinline__ void partial_prefix(const int* __restrict__ partial, int G, int* lds,
+                                               int* pre, int* grand) {
+  int p = 0, a = 0;
+  for (int j = threadIdx.x; j < G; j += blockDim.x) {
+    const int v = partial[j];
+    a += v;
+    p += j < static_cast<int>(blockIdx.x) ? v : 0;
+  }
+  block_excl_scan(p, lds, pre);
+  block_excl_scan(a, lds, grand);
 }
 
-// out[i] = exclusive prefix; out[len] = grand total.  zero_in clears `in`
-// (the cell histogram is reused by the next substep).  Each thread takes
-// four consecutive elements per block scan.
+// out[i] = exclusive prefix; out[len] = grand total (and so for in2/out2).
+// zero_in clears `in` (the cell histogram is reused by the next substep).
+// Each workgroup first sums the partials of the workgroups before it, then
+// scans its own range, each thread taking four consecutive elements per
+// block scan.
 __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
+                                                   const int* __restrict__ in2,
                                                    const int* __restrict__ d_len,
                                                    const int* __restrict__ partial,
-                                                   int* __restrict__ out, int zero_in,
+                                                   int* __restrict__ out, int* __restrict__ out2,
+                                                   int zero_in,
                                                    const unsigned* __restrict__ flags) {
   __shared__ int lds[8];
   if (fatal(flags)) return;
@@ -475,30 +493,41 @@ __global__ __launch_bounds__(256) void k_scan_down(int* __restrict__ in,
   const int chunk = (len + G - 1) / G;
   const int b0 = blockIdx.x * chunk;
   const int b1 = min(len, b0 + chunk);
-  int run = partial[blockIdx.x];
+  int run, grand, run2 = 0, grand2 = 0;
+  partial_prefix(partial, G, lds, &run, &grand);
+  if (in2) partial_prefix(partial + G, G, lds, &run2, &grand2);
   constexpr int kPer = 4;
   for (int t0 = b0; t0 < b1; t0 += kPer * static_cast<int>(blockDim.x)) {
     const int i0 = t0 + kPer * static_cast<int>(threadIdx.x);
-    int v[kPer];
-    int sum = 0;
+    int v[kPer], w[kPer];
+    int sum = 0, sum2 = 0;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       v[q] = (i0 + q < b1) ? in[i0 + q] : 0;
+      w[q] = (in2 && i0 + q < b1) ? in2[i0 + q] : 0;
       sum += v[q];
+      sum2 += w[q];
     }
-    int total;
+    int total, total2 = 0;
     int e = run + block_excl_scan(sum, lds, &total);
+    int e2 = in2 ? run2 + block_excl_scan(sum2, lds, &total2) : 0;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       if (i0 + q < b1) {
         out[i0 + q] = e;
         if (zero_in) in[i0 + q] = 0;
+        if (in2) out2[i0 + q] = e2;
       }
       e += v[q];
+      e2 += w[q];
     }
     run += total;
+    run2 += total2;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = partial[G];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[len] = grand;
+    if (in2) out2[len] = grand2;
+  }
 }
 
 // ---- stable radix sort of the slots by (cell key, original id)
@@ -1996,35 +2025,43 @@ __global__ __launch_bounds__(1024) void k_vort_apply(StepConsts c, const float4*
 // after each pass (k_run_fill).
 
 // flag[s] = 1 when slot s starts a run.
-__global__ __launch_bounds__(1024) void k_run_flags(StepConsts c, const float4* __restrict__ posB,
-                                                    const float4* __restrict__ velB,
-                                                    const int* __restrict__ slot_key,
-                                                    int* __restrict__ flag,
-                                                    const unsigned* __restrict__ flags) {
+// Per slot: run[s] = 1 when s starts a run (identical consecutive slots of
+// one cell: bitwise equal position and velocity), cell[s] = 1 when s starts
+// a (non-empty) cell.
+__global__ __launch_bounds__(1024) void k_slot_flags(StepConsts c, const float4* __restrict__ posB,
+                                                     const float4* __restrict__ velB,
+                                                     const int* __restrict__ slot_key,
+                                                     int* __restrict__ run,
+                                                     int* __restrict__ cell,
+                                                     const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
-  bool head = s == 0;
-  if (!head)
-    head = slot_key[s] != slot_key[s - 1] || !same_bits(ld3(posB, s), ld3(posB, s - 1)) ||
-           !same_bits(ld3(velB, s), ld3(velB, s - 1));
-  flag[s] = head ? 1 : 0;
+  const bool cell_head = s == 0 || slot_key[s] != slot_key[s - 1];
+  const bool run_head = cell_head || !same_bits(ld3(posB, s), ld3(posB, s - 1)) ||
+                        !same_bits(ld3(velB, s), ld3(velB, s - 1));
+  run[s] = run_head ? 1 : 0;
+  cell[s] = cell_head ? 1 : 0;
 }
 
-// From the scanned flags: the list of run heads, and each slot's run number
-// (in place of its flag).  Slots that are not heads store no neighbor list.
-__global__ __launch_bounds__(1024) void k_run_list(StepConsts c, int* __restrict__ run_no,
-                                                   const int* __restrict__ excl,
-                                                   int* __restrict__ rep,
-                                                   long long* __restrict__ noff,
-                                                   int* __restrict__ nwords,
-                                                   int* __restrict__ ncount,
-                                                   const unsigned* __restrict__ flags) {
+// From the scanned flags: the list of run heads and each slot's run number,
+// the first slot of each non-empty cell and each slot's cell number (in
+// place of the flags).  Slots that are not run heads store no neighbor list.
+__global__ __launch_bounds__(1024) void k_slot_lists(StepConsts c, int* __restrict__ run_no,
+                                                     const int* __restrict__ run_excl,
+                                                     int* __restrict__ rep,
+                                                     int* __restrict__ cell_no,
+                                                     const int* __restrict__ cell_excl,
+                                                     int* __restrict__ first,
+                                                     long long* __restrict__ noff,
+                                                     int* __restrict__ nwords,
+                                                     int* __restrict__ ncount,
+                                                     const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
   const int head = run_no[s];
-  const int e = excl[s];
+  const int e = run_excl[s];
   if (head) {
     rep[e] = s;
   } else {
@@ -2033,44 +2070,10 @@ __global__ __launch_bounds__(1024) void k_run_list(StepConsts c, int* __restrict
     ncount[s] = 0;
   }
   run_no[s] = e + head - 1;
-}
-
-// run_end[s]: one past the last slot of s's run (the next run's first slot).
-__global__ __launch_bounds__(1024) void k_run_ends(StepConsts c, const int* __restrict__ run_no,
-                                                   const int* __restrict__ excl,
-                                                   const int* __restrict__ rep,
-                                                   int* __restrict__ run_end,
-                                                   const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
-  const int r = run_no[s] + 1;
-  run_end[s] = r < excl[c.n] ? rep[r] : c.n;
-}
-
-// flag[s] = 1 when slot s starts a (non-empty) cell.
-__global__ __launch_bounds__(1024) void k_cell_flags(StepConsts c, const int* __restrict__ slot_key,
-                                                     int* __restrict__ flag,
-                                                     const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
-  flag[s] = (s == 0 || slot_key[s] != slot_key[s - 1]) ? 1 : 0;
-}
-
-// From the scanned flags: each non-empty cell's first slot, and each slot's
-// cell number (in place of its flag).
-__global__ __launch_bounds__(1024) void k_cell_list(StepConsts c, int* __restrict__ cell_no,
-                                                    const int* __restrict__ excl,
-                                                    int* __restrict__ first,
-                                                    const unsigned* __restrict__ flags) {
-  if (fatal(flags)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
-  const int head = cell_no[s];
-  const int e = excl[s];
-  if (head) first[e] = s;
-  cell_no[s] = e + head - 1;
+  const int ch = cell_no[s];
+  const int ce = cell_excl[s];
+  if (ch) first[ce] = s;
+  cell_no[s] = ce + ch - 1;
 }
 
 __device__ __forceinline__ bool aligned_cell(const StepConsts& c, int a, int b) {
@@ -2129,7 +2132,8 @@ __global__ __launch_bounds__(1024) void k_cell_lanes(StepConsts c, const int* __
 
 // lane_rep: each run's head slot at its lane (off1/off2: the cells' scanned
 // lane counts); padding lanes of aligned cells get -1.  The lanes in use are
-// [0, lane_info[0]) and [lane_info[1], lane_info[2]).
+// [0, lane_info[0]) and [lane_info[1], lane_info[2]).  Also run_end[s], one
+// past the last slot of s's run.
 __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __restrict__ run_no,
                                                    const int* __restrict__ cell_no,
                                                    const int* __restrict__ first,
@@ -2142,11 +2146,18 @@ __global__ __launch_bounds__(1024) void k_lane_map(StepConsts c, const int* __re
                                                    long long lane_cap,
                                                    int* __restrict__ lane_rep,
                                                    int* __restrict__ lane_info,
+                                                   const int* __restrict__ rep,
+                                                   int* __restrict__ run_end,
                                                    const int* __restrict__ in_tile,
                                                    const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
+  {
+    // one past the last slot of s's run: the next run's first slot
+    const int r = run_no[s] + 1;
+    run_end[s] = r < *runs_total ? rep[r] : c.n;
+  }
   const int C = excl[c.n];
   // dense and heavy cells first (their waves start first), packed runs after
   const int total2 = off2[C];  // a multiple of 64
@@ -2326,11 +2337,10 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.flags);
   }));
   FH_TRY(L("scan", [&] {
-    k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
-                                                 c.flags);
-    k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
-    k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, c.total_cells, c.scan_partial,
-                                               c.start, 1, c.flags);
+    k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells,
+                                                 c.scan_partial, c.flags);
+    k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells, c.scan_partial,
+                                               c.start, nullptr, 1, c.flags);
   }));
   // keys lie below the cell table's capacity: 8 bits per pass
   int key_bits = 1;
@@ -2343,11 +2353,11 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
       const int i = q & 1, o = i ^ 1;
       k_radix_hist<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], 8 * q, c.radix_hist,
                                                    c.flags);
-      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, c.radix_len, c.scan_partial,
-                                                   c.flags);
-      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
-      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, c.radix_len, c.scan_partial,
-                                                 c.radix_off, 0, c.flags);
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, nullptr, c.radix_len,
+                                                   c.scan_partial, c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, nullptr, c.radix_len,
+                                                 c.scan_partial, c.radix_off, nullptr, 0,
+                                                 c.flags);
       k_radix_scatter<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], c.sort_v[i], c.sort_k[o],
                                                       c.sort_v[o], 8 * q, c.radix_off, c.flags);
     }
@@ -2366,41 +2376,31 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     }));
   if (runs)
     FH_TRY(L("runs", [&] {
-      k_run_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.posB, c.velB, c.slot_key, c.run_no,
-                                                         c.flags);
-      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.n_dev, c.scan_partial, c.flags);
-      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
-      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.n_dev, c.scan_partial, c.run_excl,
-                                                 0, c.flags);
-      k_run_list<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
-                                                        c.noff, c.nwords, c.ncount, c.flags);
-      k_run_ends<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
-                                                        c.run_end, c.flags);
-      // lanes: non-empty cells, their lane counts, offsets, the lane map
-      k_cell_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.slot_key, c.cell_no, c.flags);
-      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.cell_no, c.n_dev, c.scan_partial, c.flags);
-      k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
-      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.cell_no, c.n_dev, c.scan_partial,
-                                                 c.cell_excl, 0, c.flags);
-      k_cell_list<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_no, c.cell_excl, c.cell_first,
-                                                         c.flags);
+      // runs and non-empty cells: flags, their scans (one dual scan), the
+      // lists of run heads and of cells' first slots
+      k_slot_flags<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.posB, c.velB, c.slot_key,
+                                                          c.run_no, c.cell_no, c.flags);
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.cell_no, c.n_dev, c.scan_partial,
+                                                   c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.run_no, c.cell_no, c.n_dev, c.scan_partial,
+                                                 c.run_excl, c.cell_excl, 0, c.flags);
+      k_slot_lists<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.run_no, c.run_excl, c.rep_list,
+                                                          c.cell_no, c.cell_excl, c.cell_first,
+                                                          c.noff, c.nwords, c.ncount, c.flags);
+      // lanes: the cells' lane counts, their offsets (one dual scan), the lane
+      // map (which also writes each slot's run end)
       k_cell_lanes<<<blocks_for(c.n, 256), 256, 0, st>>>(k, c.cell_excl, c.cell_first, c.run_no,
                                                           c.slot_cell, c.grid, c.start, c.seg,
                                                           c.lanes1, c.lanes2,
                                                           tiles ? c.in_tile : nullptr, c.flags);
-      for (int q = 0; q < 2; ++q) {
-        int* lanes = q == 0 ? c.lanes1 : c.lanes2;
-        int* off = q == 0 ? c.off1 : c.off2;
-        k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(lanes, c.cell_excl + c.n, c.scan_partial,
-                                                     c.flags);
-        k_scan_partials<<<1, 1024, 0, st>>>(c.scan_partial, c.scan_blocks, c.flags);
-        k_scan_down<<<c.scan_blocks, 256, 0, st>>>(lanes, c.cell_excl + c.n, c.scan_partial, off,
-                                                   0, c.flags);
-      }
+      k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.lanes1, c.lanes2, c.cell_excl + c.n,
+                                                   c.scan_partial, c.flags);
+      k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.lanes1, c.lanes2, c.cell_excl + c.n,
+                                                 c.scan_partial, c.off1, c.off2, 0, c.flags);
       k_lane_map<<<blocks_for(c.n, 256), 256, 0, st>>>(
           k, c.run_no, c.cell_no, c.cell_first, c.cell_excl, c.lanes1, c.lanes2, c.off1, c.off2,
-          c.run_excl + c.n, c.lane_cap, c.lane_rep, c.lane_info, tiles ? c.in_tile : nullptr,
-          c.flags);
+          c.run_excl + c.n, c.lane_cap, c.lane_rep, c.lane_info, c.rep_list, c.run_end,
+          tiles ? c.in_tile : nullptr, c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
@@ -2413,7 +2413,10 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   // tile slots out) and read only what earlier passes finished.  fork() makes
   // the side stream wait for the main stream's work so far, join() the main
   // stream for the side stream's.  Both are edges of a captured graph too.
-  const bool forked = tiles && c.side != nullptr;
+  // An event record and a cross-stream wait leave the device idle for tens
+  // of microseconds (about 30 us per fork measured at 65,536 particles), so
+  // small batches, whose tile kernels take about as long, stay on one stream.
+  const bool forked = tiles && c.side != nullptr && c.n >= kForkMinParticles;
   hipStream_t tst = forked ? c.side : st;
   hipError_t fj = hipSuccess;
   auto fork = [&] {

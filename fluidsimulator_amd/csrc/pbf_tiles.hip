@@ -3,12 +3,12 @@
 // The cell table orders cells in 4x4x4-cell bricks (brick_shift 2), so a
 // brick's particles are one contiguous slot range and the 6x6x6 cells around
 // it (the brick and a one-cell halo) hold every candidate of those particles.
-// A brick of at least kTileMinParticles particles is a tile when its whole
-// halo holds at most kTileMaxHalo particles, each of its non-empty cells'
-// 27-cell neighborhoods fewer than
-// kTileMaxCand, no two of its particles coincide (the per-lane passes compute
-// a run of coincident particles once) and, found by the neighbor kernel, none
-// of its particles has more than kTileList neighbors.  For a tile:
+// A brick of enough particles (kTileMinParticles in large batches) is a tile
+// when its whole halo holds at most kTileMaxHalo particles, each of its
+// non-empty cells' 27-cell neighborhoods fewer than kTileMaxCand, in large
+// batches no two of its particles coincide (the per-lane passes compute a run
+// of coincident particles once) and, found by the neighbor kernel, none of
+// its particles has more than kTileList neighbors.  For a tile:
 //
 //   * classify (once per substep): records the tile, the LDS offset of each
 //     halo cell (the cells' particle counts, scanned in halo order z, y, x)
@@ -54,9 +54,10 @@ constexpr int kPerThread = kTileMaxHalo / kPass;  // halo records staged per thr
 // Bricks with a cell of this many candidates (27-cell particles) or more take
 // the per-lane passes: the tile neighbor build tests every candidate.
 constexpr int kTileMaxCand = 512;
-// Bricks with fewer particles take the per-lane passes: a tile's fixed cost
-// (layout, halo staging, a 512-thread workgroup) is not worth it for a few
-// particles of sparse spray.
+// In batches of kForkMinParticles particles or more, bricks with fewer
+// particles take the per-lane passes: a tile's fixed cost (layout, halo
+// staging, a 512-thread workgroup) is not worth it for a few particles of
+// sparse spray, of which such batches can hold 10^5 bricks.
 constexpr int kTileMinParticles = 32;
 
 __device__ __forceinline__ bool tile_fatal(const unsigned* flags) {
@@ -121,13 +122,12 @@ __device__ __forceinline__ int tile_excl_scan(int v, int* lds, int* total) {
 
 // ------------------------------------------------------------ classify
 
-// Bricks of at least kTileMinParticles particles, appended to tile_cand
-// (tile_count[1] of them; one atomic per wavefront); the slots of smaller
-// bricks get in_tile = 0 here.  A brick of >= kTileMinParticles holds at
-// least 1/kTileMinParticles of the particles, so tile_cap entries suffice.
+// Non-empty bricks of at least min_particles particles, appended to
+// tile_cand (tile_count[1] of them; one atomic per wavefront); the slots of
+// other bricks, and of candidates beyond tile_cap, get in_tile = 0 here.
 __global__ __launch_bounds__(kThreads) void k_tile_bricks(
     const int* __restrict__ start, const int* __restrict__ total_cells, int* __restrict__ in_tile,
-    int* __restrict__ tile_count, int* __restrict__ tile_cand, int tile_cap,
+    int* __restrict__ tile_count, int* __restrict__ tile_cand, int tile_cap, int min_particles,
     const unsigned* __restrict__ flags) {
   if (tile_fatal(flags)) return;
   const int nbricks = *total_cells >> 6;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_bricks(
     if (b < nbricks) {
       const int a = start[b << 6];
       const int e = start[(b << 6) + 64];
-      big = e - a >= kTileMinParticles;
+      big = e > a && e - a >= min_particles;
       if (!big)
         for (int s = a; s < e; ++s) in_tile[s] = 0;
     }
@@ -150,7 +150,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_bricks(
     if (lane == 0) base = atomicAdd(tile_count + 1, __popcll(m));
     base = __shfl(base, 0, 64);
     const int at = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (big && at < tile_cap) tile_cand[at] = b;
+    if (big && at < tile_cap) {
+      tile_cand[at] = b;
+    } else if (big) {  // beyond the capacity: the per-lane passes
+      for (int s = start[b << 6]; s < start[(b << 6) + 64]; ++s) in_tile[s] = 0;
+    }
   }
 }
 
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
     StepConsts c, const GridDesc* __restrict__ grid, const int* __restrict__ start,
     const float4* __restrict__ posB, const int* __restrict__ tile_cand,
     int* __restrict__ in_tile, int* __restrict__ tile_count, int4* __restrict__ tile_rec,
-    int* __restrict__ tile_seg, int* __restrict__ tile_src, int tile_cap,
+    int* __restrict__ tile_seg, int* __restrict__ tile_src, int tile_cap, int reject_runs,
     const unsigned* __restrict__ flags) {
   __shared__ int hcnt[kHaloCells];
   __shared__ int hsrc[kHaloCells];
@@ -208,10 +212,11 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
         bad = cand >= kTileMaxCand;
       }
     }
-    // coincident particles (consecutive slots at one position): the per-lane
-    // passes compute a run of them once
-    for (int s = a + 1 + t; s < e && !bad; s += kThreads)
-      bad = coincide(ld3(posB, s), ld3(posB, s - 1));
+    // coincident particles (consecutive slots at one position): in large
+    // batches the per-lane passes, which compute a run of them once, take them
+    if (reject_runs)
+      for (int s = a + 1 + t; s < e && !bad; s += kThreads)
+        bad = coincide(ld3(posB, s), ld3(posB, s - 1));
     const bool ok = !__syncthreads_or(bad) && total <= kTileMaxHalo;
     if (ok && t == 0) idx_sh = atomicAdd(tile_count, 1);
     __syncthreads();
@@ -372,8 +377,8 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
     if (T.dead) continue;
     load_layout(L, ta.tile_seg, ta.tile_src, i);
     int s = T.a + static_cast<int>(threadIdx.x);
-    // a tile holds at least kTileMinParticles particles: lanes past its end
-    // load its last particle's data (in bounds) and do not use it
+    // a tile holds at least one particle: lanes past its end load its last
+    // particle's data (in bounds) and do not use it
     const int s0 = s < T.e ? s : T.e - 1;
     Nbrs nb = load_nbrs(ta, n, s0);
     X x = load(s0);
@@ -770,12 +775,19 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_apply(
 // ------------------------------------------------------------ launching
 
 hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k) {
+  // small batches tile every brick they can (coincident particles
+  // included): their per-lane kernels then have no lanes, whose latency
+  // costs each pass about 20 us for any number of them, and a few small
+  // tiles cost little; large ones leave bricks of sparse spray and runs of
+  // coincident particles to the per-lane passes
+  const bool large = c.n >= kForkMinParticles;
+  const int min_particles = large ? kTileMinParticles : 1;
   k_tile_bricks<<<1024, kThreads, 0, c.stream>>>(c.start, c.total_cells, c.in_tile, c.tile_count,
                                                 c.tile_cand, static_cast<int>(c.tile_cap),
-                                                c.flags);
+                                                min_particles, c.flags);
   k_tile_classify<<<kTileGrid, kThreads, 0, c.stream>>>(
       k, c.grid, c.start, c.posB, c.tile_cand, c.in_tile, c.tile_count, c.tile_rec, c.tile_seg,
-      c.tile_src, static_cast<int>(c.tile_cap), c.flags);
+      c.tile_src, static_cast<int>(c.tile_cap), large ? 1 : 0, c.flags);
   return hipGetLastError();
 }
 
