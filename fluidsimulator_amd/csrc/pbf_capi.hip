@@ -185,7 +185,10 @@ void compute_consts_of(Ctx& c, const fluid_hip_params* params, int n_scenes) {
   k.runs = (c.wave_mode == 2 && !c.record_diag) ? 1 : 0;
   k.full_lists = c.record_diag ? 1 : 0;
   k.count_only = 0;
-  k.tiles = (k.runs && c.brick_shift == 2 && c.tile_gather != 2 && p.solver_iterations > 0) ? 1 : 0;
+  // tile mode 0 (default) tiles from kTileAutoMinParticles particles: below
+  // that a substep is bound by launches, to which the tile kernels add
+  const bool tile_mode = c.tile_gather == 1 || (c.tile_gather == 0 && c.n >= fluidhip::kTileAutoMinParticles);
+  k.tiles = (k.runs && c.brick_shift == 2 && tile_mode && p.solver_iterations > 0) ? 1 : 0;
   c.consts = k;
   c.iterations = p.solver_iterations;
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423

@@ -17,6 +17,7 @@ namespace fluidhip {
 constexpr int kTileMaxHalo = 2048;  // particles of a tile's 6^3-cell halo held in LDS
 constexpr int kTileList = 128;      // neighbors of a tile particle, at most (else per-lane)
 constexpr int kForkMinParticles = 1 << 18;  // tile kernels on the side stream from this size
+constexpr int kTileAutoMinParticles = 1 << 15;  // default tile mode: tiles from this size
 constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
 
 // What the tile kernels read besides their pass's arrays.
@@ -263,13 +264,14 @@ hipError_t enqueue_init_grid(Ctx& c);
 // The tile kernels (pbf_tiles.hip).
 hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k);
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
-                                  const float4* p0);
+                                  const float4* p0, bool store);
 hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
-                               int write_rho, hipStream_t st);
+                               int write_rho, float4* sq, hipStream_t st);
 hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,
-                              float4* Pn, bool last, bool rest, float4* vout, hipStream_t st);
+                              float4* Pn, bool last, bool rest, float4* vout, unsigned* pm,
+                              unsigned* rm, hipStream_t st);
 hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
-                             hipStream_t st);
+                             unsigned* rm, hipStream_t st);
 hipError_t enqueue_tile_vort_omega(Ctx& c, const StepConsts& k, const TileArgs& ta,
                                    hipStream_t st);
 hipError_t enqueue_tile_vort_apply(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,

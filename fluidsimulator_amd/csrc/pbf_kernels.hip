@@ -745,6 +745,7 @@ struct CellView {
   const int* __restrict__ lane_info;       // with runs: packed lanes, aligned lanes' range
   const int* __restrict__ run_end;         // with runs: per slot, one past its run's end
   const float4* __restrict__ p0;           // predicted positions p^0 (xyz) of the lists
+  const int* __restrict__ in_tile;         // without runs: slots the tile kernels compute
 };
 
 // One lane's view: its slot (the last particle for lanes past the end, so
@@ -776,8 +777,8 @@ __device__ __forceinline__ LaneCell lane_cell(const StepConsts& c, const CellVie
     L.active = slot >= 0;
     L.s = L.active ? slot : c.n - 1;
   } else {
-    L.active = s0 < c.n;
-    L.s = L.active ? s0 : c.n - 1;
+    L.active = s0 < c.n && !(cv.in_tile && cv.in_tile[s0]);
+    L.s = s0 < c.n ? s0 : c.n - 1;
   }
   L.key = cv.slot_key[L.s];
   L.cell = cv.slot_cell[L.s];
@@ -2292,17 +2293,17 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (rec) {
     k.runs = 0;
     k.full_lists = 1;
-    k.tiles = 0;
   }
   const bool runs = k.runs != 0;
-  // sparse bricks through the LDS-tiled passes (needs the lane map of runs)
-  const bool tiles = runs && k.tiles != 0 && c.tile_list != nullptr;
+  // sparse bricks through the LDS-tiled passes: the lane map of runs leaves
+  // their slots out, or without runs (a recorded substep) the lanes skip them
+  const bool tiles = k.tiles != 0 && c.tile_list != nullptr;
   if (!tiles) k.tiles = 0;
   const TileArgs ta{c.slot_cell, c.tile_count, static_cast<int>(c.tile_cap), c.tile_rec,
                     c.tile_seg,  c.tile_src,   c.tile_list, c.tile_cnt};
   const CellView cv{c.slot_cell, c.slot_key, c.grid,      c.start,
                     c.seg,       c.lane_rep, c.lane_info, runs ? c.run_end : nullptr,
-                    PK(0)};
+                    PK(0),       (tiles && !runs) ? c.in_tile : nullptr};
   c.lists_partial = !rec && k.coop && !k.full_lists;
   c.last_runs = runs;
   auto fill = [&](float4* a, const float4* ids) {
@@ -2372,7 +2373,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   if (tiles)
     FH_TRY(L("tiles", [&] {
       (void)enqueue_tile_classify(c, k);
-      (void)enqueue_tile_neighbors(c, k, ta, PK(0));
+      (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr);
     }));
   if (runs)
     FH_TRY(L("runs", [&] {
@@ -2446,7 +2447,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("lambda", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0, tst);
+        (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0, sq, tst);
       }
       k_lambda<<<nl, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
       fill(P, nullptr);
@@ -2456,7 +2457,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("delta", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1, tst);
+        (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1, pm,
+                                 last && rest_in_delta ? rm : nullptr, tst);
       }
       if (!last)
         k_delta<true, false, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
@@ -2499,7 +2501,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("xsph", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_xsph(c, k, ta, rest, tst);
+        (void)enqueue_tile_xsph(c, k, ta, rest, rest && rec ? rec->RM : nullptr, tst);
       }
       if (rest)
         k_xsph<true><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,

@@ -93,7 +93,8 @@ __device__ __forceinline__ void brick_origin(int b, const GridDesc& g, int& rx, 
   rz = (local / (g.nbx * g.nby)) * kEdge;
 }
 
-// Exclusive scan of v over a workgroup of kThreads (lds: kThreads/64 + 1 ints).
+// Exclusive scan of v over a workgroup of kN threads (lds: kN/64 + 1 ints).
+template <int kN = kThreads>
 __device__ __forceinline__ int tile_excl_scan(int v, int* lds, int* total) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -106,16 +107,16 @@ __device__ __forceinline__ int tile_excl_scan(int v, int* lds, int* total) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int run = 0;
-    for (int i = 0; i < kThreads / 64; ++i) {
+    for (int i = 0; i < kN / 64; ++i) {
       const int t = lds[i];
       lds[i] = run;
       run += t;
     }
-    lds[kThreads / 64] = run;
+    lds[kN / 64] = run;
   }
   __syncthreads();
   const int r = lds[w] + inc - v;
-  *total = lds[kThreads / 64];
+  *total = lds[kN / 64];
   __syncthreads();
   return r;
 }
@@ -404,18 +405,24 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
 // each cell by ascending slot (= ascending id); neighbors are the other
 // candidates j with near0(p^0_i, p^0_j).  A tile with a particle of more
 // than kTileList neighbors is dead: its slots go back to the per-lane passes
-// (in_tile = 0) and the passes skip it.
+// (in_tile = 0) and the passes skip it.  With nbr (a recorded substep) the
+// live tiles' lists are also stored there as slots (noff, nwords), as the
+// per-lane neighbor kernel stores its lists.
 __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     StepConsts c, TileArgs ta, const float4* __restrict__ p0, int4* __restrict__ tile_rec,
     int* __restrict__ in_tile, int* __restrict__ ncount, long long* __restrict__ noff,
-    int* __restrict__ nwords, const unsigned* __restrict__ flags) {
+    int* __restrict__ nwords, int* __restrict__ nbr, long long nbr_cap,
+    unsigned long long* __restrict__ nbr_total, unsigned* __restrict__ flags) {
   __shared__ TileLayout L;
   __shared__ float X[kTileMaxHalo], Y[kTileMaxHalo], Z[kTileMaxHalo];
+  __shared__ int scan_lds[kPass / 64 + 1];
+  __shared__ long long base_sh;
   if (tile_fatal(flags)) return;
   const int nt = tiles_of(ta.tile_count, ta.tile_cap);
   const int per = (nt + 7) >> 3;
   const int qend = c.xcd_remap ? per << 3 : nt;
   uint4* list = const_cast<uint4*>(ta.tile_list);
+  // nbr_total counts this substep's stored words from zero (grid setup)
   int* cnt_out = const_cast<int*>(ta.tile_cnt);
   for (int q = blockIdx.x; q < qend; q += gridDim.x) {
     const int i = c.xcd_remap ? (q & 7) * per + (q >> 3) : q;
@@ -475,6 +482,43 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     if (__syncthreads_or(over)) {
       for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) in_tile[s] = 0;
       if (threadIdx.x == 0) tile_rec[2 * i + 1].w = -1;
+    } else if (nbr) {
+      // a recorded substep: the lists also go to the shared list buffer as
+      // plain slot words, in the same order, for the adjoint; one allocation
+      // per tile
+      int mine = 0;
+      for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) mine += cnt_out[s];
+      int total;
+      const int pre = tile_excl_scan<kPass>(mine, scan_lds, &total);
+      if (threadIdx.x == 0)
+        base_sh = static_cast<long long>(
+            atomicAdd(nbr_total, static_cast<unsigned long long>(total)));
+      __syncthreads();
+      long long off = base_sh + pre;
+      if (base_sh + total > nbr_cap) {
+        if (threadIdx.x == 0) atomicOr(flags, kFlagNeighbors);
+      } else {
+        for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) {
+          int lx, ly, lz;
+          local_cell(ta.slot_cell[s], T.cx0, T.cy0, T.cz0, lx, ly, lz);
+          const int own = ((lz + 1) * kHalo + ly + 1) * kHalo + lx + 1;
+          const int self = L.seg[own] + (s - L.src[own]);
+          const float3 me = make_float3(X[self], Y[self], Z[self]);
+          int k = 0;
+          for (int row = 0; row < 9; ++row) {
+            const int hc = ((lz + row / 3) * kHalo + ly + row % 3) * kHalo + lx;
+            for (int cc = hc; cc < hc + 3; ++cc) {
+              const int lo = L.seg[cc], hi = L.seg[cc + 1];
+              for (int r = lo; r < hi; ++r)
+                if (r != self && near0(me, make_float3(X[r], Y[r], Z[r]), c))
+                  nbr[off + k++] = L.src[cc] + (r - lo);
+            }
+          }
+          noff[s] = off;
+          nwords[s] = k;
+          off += k;
+        }
+      }
     }
     __syncthreads();  // LDS is reused by the next tile
   }
@@ -483,11 +527,13 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
 // ------------------------------------------------------------ lambda
 
 // lambda pass (core.cpp:281-329) of the tiles; the per-lane term and its
-// accumulation order of k_lambda.
+// accumulation order of k_lambda.  A recorded substep also stores the
+// gradient sum S and the denominator sum Q (sq_out).
 __global__ __launch_bounds__(kPass) void k_tile_lambda(StepConsts c, TileArgs ta,
                                                        float4* __restrict__ P,
                                                        float* __restrict__ rho_out,
                                                        int write_rho,
+                                                       float4* __restrict__ sq_out,
                                                        const unsigned* __restrict__ flags) {
   __shared__ TileLayout L;
   __shared__ float X[kTileMaxHalo], Y[kTileMaxHalo], Z[kTileMaxHalo];
@@ -529,19 +575,23 @@ __global__ __launch_bounds__(kPass) void k_tile_lambda(StepConsts c, TileArgs ta
         sg2 += gix * gix + giy * giy + giz * giz;
         reinterpret_cast<float*>(P + s)[3] = -C / (sg2 + c.eps);
         if (write_rho) rho_out[s] = rho;
+        if (sq_out) sq_out[s] = make_float4(gsx, gsy, gsz, sg2);
       });
 }
 
 // ------------------------------------------------------------ delta
 
 // delta pass with plane projection and apply (core.cpp:334-407) of the
-// tiles; kLast also commits (core.cpp:414-421), kRest fuses restitution.
+// tiles; kLast also commits (core.cpp:414-421), kRest fuses restitution.  A
+// recorded substep also stores the planes that pushed (pmask) and the
+// restitution decisions (rmask).
 template <bool kLast, bool kRest>
 __global__ __launch_bounds__(kPass) void k_tile_delta(
     StepConsts c, TileArgs ta, const float4* __restrict__ P, float4* __restrict__ Pnext,
     const ScenePlanes* __restrict__ sp,
     const float4* __restrict__ posB, float4* __restrict__ posA, float4* __restrict__ vout,
-    const float* __restrict__ rho, const unsigned* __restrict__ flags) {
+    const float* __restrict__ rho, unsigned* __restrict__ pmask, unsigned* __restrict__ rmask,
+    const unsigned* __restrict__ flags) {
   __shared__ TileLayout L;
   __shared__ float X[kTileMaxHalo], Y[kTileMaxHalo], Z[kTileMaxHalo], W[kTileMaxHalo];
   if (tile_fatal(flags)) return;
@@ -580,13 +630,15 @@ __global__ __launch_bounds__(kPass) void k_tile_delta(
         ddy *= c.inv_density;
         ddz *= c.inv_density;
         const ScenePlanes& plane = sp[T.sc];
+        unsigned pushed = 0;
         if (plane.n_planes > 0) {
           float qx = pi.x + ddx, qy = pi.y + ddy, qz = pi.z + ddz;
-          project_planes(qx, qy, qz, plane);
+          pushed = project_planes(qx, qy, qz, plane);
           ddx = qx - pi.x;
           ddy = qy - pi.y;
           ddz = qz - pi.z;
         }
+        if (pmask) pmask[s] = pushed;
         const float nx = pi.x + ddx, ny = pi.y + ddy, nz = pi.z + ddz;  // core.cpp:404-406
         if (!kLast) {
           Pnext[s] = make_float4(nx, ny, nz, 0.0f);
@@ -597,7 +649,10 @@ __global__ __launch_bounds__(kPass) void k_tile_delta(
         float vy = (ny - x.y) / c.dt;
         float vz = (nz - x.z) / c.dt;
         posA[s] = make_float4(nx, ny, nz, x.w);
-        if (kRest) restitute(vx, vy, vz, nx, ny, nz, plane);
+        if (kRest) {
+          const unsigned rm = restitute(vx, vy, vz, nx, ny, nz, plane);
+          if (rmask) rmask[s] = rm;
+        }
         vout[s] = make_float4(vx, vy, vz, rho[s]);
       });
 }
@@ -613,7 +668,7 @@ struct PosVel {
 template <bool kRest>
 __global__ __launch_bounds__(kPass) void k_tile_xsph(
     StepConsts c, TileArgs ta, const float4* __restrict__ pos, const float4* __restrict__ vin,
-    float4* __restrict__ vout, const ScenePlanes* __restrict__ sp,
+    float4* __restrict__ vout, const ScenePlanes* __restrict__ sp, unsigned* __restrict__ rmask,
     const unsigned* __restrict__ flags) {
   __shared__ TileLayout L;
   __shared__ float X[kTileMaxHalo], Y[kTileMaxHalo], Z[kTileMaxHalo];
@@ -651,7 +706,10 @@ __global__ __launch_bounds__(kPass) void k_tile_xsph(
         float vx = vi.x + c.visc_c * ax;
         float vy = vi.y + c.visc_c * ay;
         float vz = vi.z + c.visc_c * az;
-        if (kRest) restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[T.sc]);
+        if (kRest) {
+          const unsigned rm = restitute(vx, vy, vz, xi.x, xi.y, xi.z, sp[T.sc]);
+          if (rmask) rmask[s] = rm;
+        }
         vout[s] = make_float4(vx, vy, vz, 0.0f);
       });
 }
@@ -792,41 +850,42 @@ hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k) {
 }
 
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
-                                  const float4* p0) {
-  k_tile_neighbors<<<kTileGrid, kPass, 0, c.stream>>>(k, ta, p0, c.tile_rec, c.in_tile,
-                                                      c.ncount, c.noff, c.nwords, c.flags);
+                                  const float4* p0, bool store) {
+  k_tile_neighbors<<<kTileGrid, kPass, 0, c.stream>>>(
+      k, ta, p0, c.tile_rec, c.in_tile, c.ncount, c.noff, c.nwords, store ? c.nbr : nullptr,
+      c.nbr_cap, c.nbr_total, c.flags);
   return hipGetLastError();
 }
 
 hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
-                               int write_rho, hipStream_t st) {
-  k_tile_lambda<<<kTileGrid, kPass, 0, st>>>(k, ta, P, c.rho, write_rho,
-                                                     c.flags);
+                               int write_rho, float4* sq, hipStream_t st) {
+  k_tile_lambda<<<kTileGrid, kPass, 0, st>>>(k, ta, P, c.rho, write_rho, sq, c.flags);
   return hipGetLastError();
 }
 
 hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,
-                              float4* Pn, bool last, bool rest, float4* vout, hipStream_t st) {
+                              float4* Pn, bool last, bool rest, float4* vout, unsigned* pm,
+                              unsigned* rm, hipStream_t st) {
   if (!last)
     k_tile_delta<false, false><<<kTileGrid, kPass, 0, st>>>(
-        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
+        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, pm, rm, c.flags);
   else if (rest)
     k_tile_delta<true, true><<<kTileGrid, kPass, 0, st>>>(
-        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
+        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, pm, rm, c.flags);
   else
     k_tile_delta<true, false><<<kTileGrid, kPass, 0, st>>>(
-        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, c.flags);
+        k, ta, P, Pn, c.planes, c.posB, c.posA, vout, c.rho, pm, rm, c.flags);
   return hipGetLastError();
 }
 
 hipError_t enqueue_tile_xsph(Ctx& c, const StepConsts& k, const TileArgs& ta, bool rest,
-                             hipStream_t st) {
+                             unsigned* rm, hipStream_t st) {
   if (rest)
-    k_tile_xsph<true><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA,
-                                                           c.planes, c.flags);
+    k_tile_xsph<true><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA, c.planes, rm,
+                                                   c.flags);
   else
-    k_tile_xsph<false><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA,
-                                                            c.planes, c.flags);
+    k_tile_xsph<false><<<kTileGrid, kPass, 0, st>>>(k, ta, c.posA, c.velB, c.velA, c.planes,
+                                                    nullptr, c.flags);
   return hipGetLastError();
 }
 

@@ -101,8 +101,9 @@ typedef struct fluid_hip_config {
                                  [n, 3n + 64]; a layout that does not fit
                                  packs one lane per run */
   int32_t tile_gather;        /* passes of sparse 4x4x4-cell bricks through LDS tiles:
-                                 0 default (on), 1 on, 2 off; used with brick edge 4 and
-                                 gather mode 2 (results are the same either way) */
+                                 0 default (on from 32,768 particles), 1 on, 2 off; used
+                                 with brick edge 4 and gather mode 2 (results are the
+                                 same either way) */
 } fluid_hip_config;
 
 typedef struct fluid_hip_ctx fluid_hip_ctx;

@@ -69,10 +69,10 @@ def fp64_target(p, pos, vtg, steps):
                        dtype=np.float64)["pos"]
 
 
-def hip_gradients(p, pos, vel, steps, target, every=10):
+def hip_gradients(p, pos, vel, steps, target, every=10, tile_gather=0):
     """L = mean_i |x_T,i - target_i|^2 and its gradients through the C ABI."""
     n = pos.shape[1]
-    with hip.HipSolver(p, [n]) as s:
+    with hip.HipSolver(p, [n], tile_gather=tile_gather) as s:
         s.upload(pos, vel)
         s.forward_record(steps, every)
         out = s.download()
@@ -271,6 +271,28 @@ def test_checkpointing_is_bitwise_invisible():
         got = hip_gradients(p, pos, vel, steps, tgt, every=every)
         for a, b in zip(got[1:], ref[1:]):
             assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32)), every
+
+
+@pytest.mark.parametrize("kind,restitution", [("reference", True), ("stated", True),
+                                               ("smooth", False)])
+def test_recorded_tiles_give_the_same_gradients(kind, restitution):
+    """Recorded substeps with LDS tiles (their lists stored for the reverse
+    pass, their iterates, S/Q sums and plane decisions written by the tile
+    kernels) give bitwise the gradients of recorded substeps without tiles,
+    with and without checkpoints."""
+    p, pos, vel, vtg = make_scene(kind, (16, 12, 10), restitution=restitution)
+    steps = 12
+    tgt = fp64_target(p, pos, vtg, steps)
+    off = hip_gradients(p, pos, vel, steps, tgt, every=steps, tile_gather=2)
+    for every in (steps, 5):
+        on = hip_gradients(p, pos, vel, steps, tgt, every=every, tile_gather=1)
+        assert on[0] == off[0]
+        for a, b in zip(on[1:], off[1:]):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32)), every
+    with hip.HipSolver(p, [pos.shape[1]], tile_gather=1) as s:
+        s.upload(pos, vel)
+        s.step(1)
+        assert s.stats()["tiles"] > 0
 
 
 def test_backward_repeatable_and_state_restored():

@@ -132,11 +132,15 @@ def test_batched_scenes_with_tiles(graph):
 
 
 def test_brick_edges_without_tiles_agree():
-    """Tiles need 4-cell bricks; other brick edges take the per-lane passes."""
+    """Tiles need 4-cell bricks; other brick edges take the per-lane passes.
+    The default tile mode tiles from 32,768 particles, so this 3,840-particle
+    scene has none; asked for, it has them."""
     sc = scenes.dam_break(seed=12, dims=(20, 16, 12), flags=True)
-    ref, _, st4 = run(sc.params, sc.state.pos, sc.state.vel, 4, brick_edge=4)
+    ref, _, st4 = run(sc.params, sc.state.pos, sc.state.vel, 4, brick_edge=4, tile_gather=1)
     assert st4["tiles"] > 0
+    _, _, st0 = run(sc.params, sc.state.pos, sc.state.vel, 1, brick_edge=4)
+    assert st0["tiles"] == 0
     for edge in (1, 2, 8):
-        r, _, st = run(sc.params, sc.state.pos, sc.state.vel, 4, brick_edge=edge)
+        r, _, st = run(sc.params, sc.state.pos, sc.state.vel, 4, brick_edge=edge, tile_gather=1)
         assert st["tiles"] == 0
         assert same_bits(r.pos, ref.pos) and same_bits(r.vel, ref.vel)
