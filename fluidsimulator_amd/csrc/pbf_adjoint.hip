@@ -363,6 +363,35 @@ __global__ __launch_bounds__(1024) void k_adj_gravity(StepConsts c, const int* _
   }
 }
 
+// The tape copy of a recorded substep, on the device: the substep's list
+// length is a device value, so no host round trip sits between substeps.
+__global__ __launch_bounds__(256) void k_tape_copy(
+    int n, const float4* __restrict__ posB, const float4* __restrict__ posA,
+    const float4* __restrict__ velB, const long long* __restrict__ noff,
+    const int* __restrict__ nwords, const int* __restrict__ ncount, const int* __restrict__ nbr,
+    const unsigned long long* __restrict__ nbr_total, float4* __restrict__ X,
+    float4* __restrict__ PI, float4* __restrict__ VC, long long* __restrict__ tnoff,
+    int* __restrict__ tnwords, int* __restrict__ tncount, int* __restrict__ tnbr, long long tcap,
+    unsigned long long* __restrict__ need, const unsigned* __restrict__ flags) {
+  if (*flags & kFlagFatal) return;  // the substep failed and will run again
+  const unsigned long long words = *nbr_total;
+  if (words > static_cast<unsigned long long>(tcap)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(need, words);
+    return;
+  }
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    X[i] = posB[i];
+    PI[i] = posA[i];
+    if (VC) VC[i] = velB[i];
+    tnoff[i] = noff[i];
+    tnwords[i] = nwords[i];
+    tncount[i] = ncount[i];
+  }
+  const long long w = static_cast<long long>(words);
+  for (long long i = blockIdx.x * blockDim.x + threadIdx.x; i < w; i += stride) tnbr[i] = nbr[i];
+}
+
 __global__ void k_soa_to_f4(int n, const float* __restrict__ soa, float4* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -434,6 +463,15 @@ hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b
                                      b.GV, c.seg, c.planes);
   }
   k_adj_gravity<<<c.n_scenes, 1024, 0, st>>>(k, c.seg, b.PB, b.gacc);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_tape_copy(Ctx& c, SubTape& t, unsigned long long* need) {
+  if (c.n == 0) return hipSuccess;
+  k_tape_copy<<<1024, 256, 0, c.stream>>>(c.n, c.posB, c.posA, c.velB, c.noff, c.nwords, c.ncount,
+                                          c.nbr, c.nbr_total, t.X, t.PI, c.xsph ? t.VC : nullptr,
+                                          t.noff, t.nwords, t.ncount, t.nbr, t.nbr_cap, need,
+                                          c.flags);
   return hipGetLastError();
 }
 

@@ -467,6 +467,7 @@ void free_adj(Ctx& c) {
   for (float4** p : {&b.GX, &b.GV, &b.U, &b.PB, &b.XB, &b.W, &b.LB}) dfree(*p);
   dfree(b.RB);
   dfree(b.gacc);
+  dfree(b.tape_need);
   dfree(b.stage);
   delete c.adj;
   c.adj = nullptr;
@@ -500,48 +501,59 @@ int alloc_tape(Ctx& c, SubTape& t) {
   return FLUID_HIP_OK;
 }
 
-// One substep with its recording (replayed after capacity growth like any
-// other substep), then a copy of the input positions, the committed state
-// and the neighbor lists into the tape.
-int run_recorded(Ctx& c, SubTape& t) {
-  const fluidhip::SubstepRec rec{t.Pk.data(), t.SQk.data(), t.Mk.data(), t.RM};
-  DevStatus st{};
+int load_state(Ctx& c, const float4* pos, const float4* vel);
+
+// Segment sg of a recorded rollout: `len` recorded substeps from the state
+// the context holds (the segment's checkpoint), each followed by its tape
+// copy on the device, and one status read at the end.  When a substep ran
+// out of table capacity or a tape lacked room for its lists, the tables and
+// tapes grow and the whole segment runs again from its checkpoint.
+int record_segment(Ctx& c, AdjRecord& a, int sg, int len) {
+  auto& b = a.buf;
+  if (!b.tape_need) HIP_TRY(dalloc(&b.tape_need, 1), "hipMalloc(tape need)");
   int rc;
-  for (int attempt = 0;; ++attempt) {
-    if (attempt >= 16) return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
+  for (int attempt = 0; attempt < 16; ++attempt) {
+    if (attempt > 0) {
+      if ((rc = load_state(c, a.ck_pos[sg], a.ck_vel[sg]))) return rc;
+      c.time = a.ck_time[sg];
+    }
     HIP_TRY(fluidhip::enqueue_reset_flags(c), "reset flags");
+    HIP_TRY(hipMemsetAsync(b.tape_need, 0, sizeof(unsigned long long), c.stream), "reset need");
     c.prof.step = 0;
-    hipError_t e = fluidhip::enqueue_substep(c, &rec);
-    if (e != hipSuccess) return hip_fail(e, "recorded substep launch");
+    for (int j = 0; j < len; ++j) {
+      SubTape& t = a.tapes[j];
+      const fluidhip::SubstepRec rec{t.Pk.data(), t.SQk.data(), t.Mk.data(), t.RM};
+      hipError_t e = fluidhip::enqueue_substep(c, &rec);
+      if (e != hipSuccess) return hip_fail(e, "recorded substep launch");
+      e = fluidhip::enqueue_tape_copy(c, t, b.tape_need);
+      if (e != hipSuccess) return hip_fail(e, "tape copy");
+    }
+    unsigned long long need = 0;
+    HIP_TRY(hipMemcpyAsync(&need, b.tape_need, sizeof(need), hipMemcpyDeviceToHost, c.stream),
+            "read tape need");
+    DevStatus st{};
     if ((rc = read_status(c, st))) return rc;
-    if ((rc = collect_profile(c, (st.flags & fluidhip::kFlagFatal) ? 0 : INT_MAX))) return rc;
-    if (!(st.flags & fluidhip::kFlagFatal)) break;
+    const bool fatal = (st.flags & fluidhip::kFlagFatal) != 0;
+    if ((rc = collect_profile(c, fatal || need ? 0 : INT_MAX))) return rc;
+    if (!fatal && need == 0) {
+      for (int j = 0; j < len; ++j) c.time += c.params[0].dt;
+      c.stepped = true;
+      return FLUID_HIP_OK;
+    }
     if (st.flags & fluidhip::kFlagRange)
       return fail(FLUID_HIP_ERANGE, "non-finite or out-of-range predicted position");
-    if ((rc = grow_tables(c, st))) return rc;
+    if (fatal && (rc = grow_tables(c, st))) return rc;
+    if (need) {
+      const long long cap = static_cast<long long>(need + need / 4 + 1024);
+      for (auto& t : a.tapes)
+        if (t.nbr_cap < cap) {
+          dfree(t.nbr);
+          HIP_TRY(dalloc(&t.nbr, static_cast<size_t>(cap)), "hipMalloc(tape neighbor lists)");
+          t.nbr_cap = cap;
+        }
+    }
   }
-  const size_t n = static_cast<size_t>(c.n);
-  const long long words = static_cast<long long>(st.nbr_total);
-  if (words > t.nbr_cap) {
-    dfree(t.nbr);
-    const long long cap = words + words / 4 + 1024;
-    HIP_TRY(dalloc(&t.nbr, static_cast<size_t>(cap)), "hipMalloc(tape neighbor lists)");
-    t.nbr_cap = cap;
-  }
-  hipStream_t s = c.stream;
-  const auto D2D = hipMemcpyDeviceToDevice;
-  HIP_TRY(hipMemcpyAsync(t.X, c.posB, sizeof(float4) * n, D2D, s), "tape copy");
-  HIP_TRY(hipMemcpyAsync(t.PI, c.posA, sizeof(float4) * n, D2D, s), "tape copy");
-  if (c.xsph) HIP_TRY(hipMemcpyAsync(t.VC, c.velB, sizeof(float4) * n, D2D, s), "tape copy");
-  HIP_TRY(hipMemcpyAsync(t.noff, c.noff, sizeof(long long) * n, D2D, s), "tape copy");
-  HIP_TRY(hipMemcpyAsync(t.nwords, c.nwords, sizeof(int) * n, D2D, s), "tape copy");
-  HIP_TRY(hipMemcpyAsync(t.ncount, c.ncount, sizeof(int) * n, D2D, s), "tape copy");
-  if (words > 0)
-    HIP_TRY(hipMemcpyAsync(t.nbr, c.nbr, sizeof(int) * static_cast<size_t>(words), D2D, s),
-            "tape copy");
-  c.time += c.params[0].dt;
-  c.stepped = true;
-  return FLUID_HIP_OK;
+  return fail(FLUID_HIP_ECAPACITY, "capacity growth did not converge");
 }
 
 int save_state(Ctx& c, float4* pos, float4* vel) {
@@ -597,6 +609,11 @@ int noalloc_scratch(int dev, unsigned char** out) {
   }
   *out = g_noalloc_ptr[dev];
   return FLUID_HIP_OK;
+}
+
+// Entries of a tile particle's neighbor list for a batch of n particles.
+int tile_list_cap_for(long long n) {
+  return n < fluidhip::kForkMinParticles ? fluidhip::kTileListSmall : fluidhip::kTileList;
 }
 
 // The context's own streams and events (the main stream only when it is not
@@ -684,7 +701,8 @@ bool noalloc_layout(Ctx& c, unsigned char* base, size_t bytes) {
   take(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   take(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   take(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
-  take(c.tile_list, static_cast<size_t>(fluidhip::kTileList / 8) * n);
+  c.tile_list_cap = tile_list_cap_for(c.n);
+  take(c.tile_list, static_cast<size_t>(c.tile_list_cap / 8) * n);
   take(c.tile_cnt, n);
   if (!ok) return false;
   off = (off + 255) & ~static_cast<size_t>(255);
@@ -887,7 +905,8 @@ int fluid_hip_create(fluid_hip_ctx** out, int32_t n_scenes, const int64_t* scene
   ALLOC(c.tile_rec, 2 * static_cast<size_t>(c.tile_cap));
   ALLOC(c.tile_seg, static_cast<size_t>(c.tile_cap) * 217);
   ALLOC(c.tile_src, static_cast<size_t>(c.tile_cap) * 216);
-  ALLOC(c.tile_list, static_cast<size_t>(fluidhip::kTileList / 8) * n);
+  c.tile_list_cap = tile_list_cap_for(c.n);
+  ALLOC(c.tile_list, static_cast<size_t>(c.tile_list_cap / 8) * n);
   ALLOC(c.tile_cnt, n);
 #undef ALLOC
   e = hipMemcpy(c.n_dev, &c.n, sizeof(int), hipMemcpyHostToDevice);
@@ -1225,7 +1244,7 @@ int fluid_hip_get_neighbors(fluid_hip_ctx* ctx, int32_t* out, int64_t cap, int64
     const size_t cap = static_cast<size_t>(c.tile_cap);
     tseg.resize(cap * 217);
     tsrc.resize(cap * 216);
-    tlist.resize(static_cast<size_t>(fluidhip::kTileList) * n);
+    tlist.resize(static_cast<size_t>(c.tile_list_cap) * n);
     tcnt.resize(n);
     HIP_TRY(hipMemcpy(tseg.data(), c.tile_seg, sizeof(int) * tseg.size(), hipMemcpyDeviceToHost),
             "read tiles");
@@ -1614,8 +1633,7 @@ int fluid_hip_forward_record(fluid_hip_ctx* ctx, int32_t nsteps, int32_t checkpo
       // plain substeps; only the last segment stays recorded
       if ((rc = fluid_hip_step(ctx, len, 0))) return rc;
     } else {
-      for (int j = 0; j < len; ++j)
-        if ((rc = run_recorded(c, a.tapes[j]))) return rc;
+      if ((rc = record_segment(c, a, sg, len))) return rc;
     }
   }
   if ((rc = save_state(c, a.fin_pos, a.fin_vel))) return rc;
@@ -1657,11 +1675,9 @@ int fluid_hip_backward(fluid_hip_ctx* ctx, const float* dpos, const float* dvel,
       // recompute the segment from its checkpoint, recording every substep
       if ((rc = load_state(c, a.ck_pos[sg], a.ck_vel[sg]))) return rc;
       c.time = a.ck_time[sg];
-      for (int j = 0; j < len; ++j) {
-        if ((rc = run_recorded(c, a.tapes[j]))) {
-          a.tape_seg = -1;
-          return rc;
-        }
+      if ((rc = record_segment(c, a, sg, len))) {
+        a.tape_seg = -1;
+        return rc;
       }
       a.tape_seg = sg;
       a.recomputed += len;

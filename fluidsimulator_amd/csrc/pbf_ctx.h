@@ -15,7 +15,11 @@ namespace fluidhip {
 
 // Brick tiles of the LDS-tiled gather passes (pbf_tiles.hip).
 constexpr int kTileMaxHalo = 2048;  // particles of a tile's 6^3-cell halo held in LDS
-constexpr int kTileList = 128;      // neighbors of a tile particle, at most (else per-lane)
+// neighbors of a tile particle, at most (else its tile goes per-lane): in
+// batches under kForkMinParticles particles, which tile their piles of
+// coincident particles too, kTileListSmall
+constexpr int kTileList = 128;
+constexpr int kTileListSmall = 256;
 constexpr int kForkMinParticles = 1 << 18;  // tile kernels on the side stream from this size
 constexpr int kTileAutoMinParticles = 1 << 15;  // default tile mode: tiles from this size
 constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
@@ -28,8 +32,9 @@ struct TileArgs {
   const int4* tile_rec;    // per tile: {a, e, halo records, scene}, {first cell x, y, z, brick}
   const int* tile_seg;     // per tile: 217 LDS offsets of the halo cells
   const int* tile_src;     // per tile: 216 first slots of the halo cells
-  const uint4* tile_list;  // kTileList / 8 x n chunks of 8 neighbors' LDS records
-  const int* tile_cnt;        // n: neighbors of a tile particle
+  const uint4* tile_list;  // list_cap / 8 x n chunks of 8 neighbors' LDS records
+  int list_cap;            // entries of a list
+  const int* tile_cnt;     // n: neighbors of a tile particle
 };
 
 struct Prof {
@@ -133,7 +138,8 @@ struct Ctx {
   int4* tile_rec = nullptr;    // 2 tile_cap: tile records
   int* tile_seg = nullptr;     // tile_cap x 217: LDS offset of each halo cell
   int* tile_src = nullptr;     // tile_cap x 216: first slot of each halo cell
-  uint4* tile_list = nullptr;  // kTileList / 8 x n: chunks of 8 neighbors' LDS records (16 bit)
+  uint4* tile_list = nullptr;  // tile_list_cap / 8 x n: chunks of 8 neighbors' LDS records (16 bit)
+  int tile_list_cap = 0;       // kTileList, or kTileListSmall for small batches
   int* tile_cnt = nullptr;        // n: neighbors of a tile particle
   long long tile_cap = 0;
 
@@ -231,6 +237,7 @@ struct AdjBuffers {
   float* RB = nullptr;   // rho-bar
   double* gacc = nullptr;
   float* stage = nullptr;  // 6n floats: SoA transfers
+  unsigned long long* tape_need = nullptr;  // list words a tape lacked room for (0: none)
 };
 
 // A recorded rollout: the state before every segment of `every` substeps,
@@ -253,6 +260,10 @@ hipError_t enqueue_adjoint_seed(Ctx& c, const float* d_dpos, const float* d_dvel
                                 const AdjBuffers& b);
 hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b);
 hipError_t enqueue_adjoint_result(Ctx& c, float* d_dpos, float* d_dvel, const AdjBuffers& b);
+// After a recorded substep (skipped when it failed): its input positions,
+// committed state, commit velocity and neighbor lists into the tape; lists
+// longer than the tape's room only raise *need to their length.
+hipError_t enqueue_tape_copy(Ctx& c, SubTape& t, unsigned long long* need);
 hipError_t enqueue_upload(Ctx& c, const float* d_pos, const float* d_vel);
 hipError_t enqueue_download(Ctx& c, float* d_pos, float* d_vel);
 hipError_t enqueue_reset_flags(Ctx& c);

@@ -2300,7 +2300,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   const bool tiles = k.tiles != 0 && c.tile_list != nullptr;
   if (!tiles) k.tiles = 0;
   const TileArgs ta{c.slot_cell, c.tile_count, static_cast<int>(c.tile_cap), c.tile_rec,
-                    c.tile_seg,  c.tile_src,   c.tile_list, c.tile_cnt};
+                    c.tile_seg,  c.tile_src,   c.tile_list, c.tile_list_cap, c.tile_cnt};
   const CellView cv{c.slot_cell, c.slot_key, c.grid,      c.start,
                     c.seg,       c.lane_rep, c.lane_info, runs ? c.run_end : nullptr,
                     PK(0),       (tiles && !runs) ? c.in_tile : nullptr};
@@ -2369,7 +2369,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                  c.slot_key, c.flags);
   }));
   // tiles and their neighbor lists come before the lane map: a tile with a
-  // particle of more than kTileList neighbors is handed back to the lanes
+  // particle of more neighbors than its list holds is handed back to the lanes
   if (tiles)
     FH_TRY(L("tiles", [&] {
       (void)enqueue_tile_classify(c, k);

@@ -8,7 +8,8 @@
 // non-empty cells' 27-cell neighborhoods fewer than kTileMaxCand, in large
 // batches no two of its particles coincide (the per-lane passes compute a run
 // of coincident particles once) and, found by the neighbor kernel, none of
-// its particles has more than kTileList neighbors.  For a tile:
+// its particles has more neighbors than a tile list holds (kTileList, or
+// kTileListSmall in small batches).  For a tile:
 //
 //   * classify (once per substep): records the tile, the LDS offset of each
 //     halo cell (the cells' particle counts, scanned in halo order z, y, x)
@@ -20,7 +21,7 @@
 //     by ascending slot (= ascending id), and candidate j is a neighbor when
 //     it is not the particle itself and |p^0_i - p^0_j|^2 < h^2 (near0, the
 //     reference's list predicate, core.cpp:205-247).  A tile with a particle
-//     of more than kTileList neighbors goes back to the per-lane passes;
+//     of more neighbors than its list holds goes back to the per-lane passes;
 //   * each gather pass (lambda, delta, XSPH, omega, eta): stages the halo's
 //     records of the pass in LDS and visits every particle's list in order,
 //     i.e. the reference's list order, with the per-lane term arithmetic of
@@ -328,8 +329,8 @@ __device__ __forceinline__ void tile_walk(const uint4* __restrict__ list, int n,
 }
 
 // A tile's slot range [a, e), halo records, scene and first cell; dead: the
-// tile went back to the per-lane passes (a particle with more than
-// kTileList neighbors).
+// tile went back to the per-lane passes (a particle with more neighbors than
+// a tile list holds).
 struct TileInfo {
   int a, e, nrec, sc;
   int cx0, cy0, cz0;
@@ -404,7 +405,7 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
 // as LDS records of the tile's halo, in the 27 cells' (dz, dy, dx) order,
 // each cell by ascending slot (= ascending id); neighbors are the other
 // candidates j with near0(p^0_i, p^0_j).  A tile with a particle of more
-// than kTileList neighbors is dead: its slots go back to the per-lane passes
+// neighbors than its list holds is dead: its slots go back to the per-lane passes
 // (in_tile = 0) and the passes skip it.  With nbr (a recorded substep) the
 // live tiles' lists are also stored there as slots (noff, nwords), as the
 // per-lane neighbor kernel stores its lists.
@@ -458,12 +459,12 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
           w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
           w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
           w3 = (w3 >> 16) | (static_cast<unsigned>(r) << 16);
-          if ((k & 7) == 7 && k < kTileList)
+          if ((k & 7) == 7 && k < ta.list_cap)
             list[static_cast<size_t>(k >> 3) * c.n + s] = make_uint4(w0, w1, w2, w3);
           ++k;
         }
       }
-      if ((k & 7) != 0 && k < kTileList) {
+      if ((k & 7) != 0 && k < ta.list_cap) {
         // the last m < 8 entries sit in the top m halves: shift them down
         for (int m = k & 7; m < 8; ++m) {
           w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
       ncount[s] = k;
       nwords[s] = -1;  // no stored list: the tile passes read their own
       noff[s] = 0;
-      over |= k > kTileList;
+      over |= k > ta.list_cap;
     }
     if (__syncthreads_or(over)) {
       for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) in_tile[s] = 0;

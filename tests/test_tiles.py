@@ -86,12 +86,12 @@ def test_tiles_next_to_piled_up_cells():
 
 @pytest.mark.parametrize("flags", [False, True])
 def test_tiles_with_long_lists_go_back_to_the_lanes(flags):
-    """A cluster of 6^3 particles at spacing 0.012 (h = 0.1): its inner
-    particles have more neighbors than the 128 a tile list holds, so its
-    brick goes back to the per-lane passes in the same substep, while the
-    lattice block at spacing 0.05 beside it stays tiles."""
+    """A cluster of 7^3 particles at spacing 0.01 (h = 0.1): its inner
+    particles have more neighbors than the 256 a tile list of a small batch
+    holds, so its brick goes back to the per-lane passes in the same substep,
+    while the lattice block at spacing 0.05 beside it stays tiles."""
     sc = scenes.dam_break(seed=9, dims=(20, 20, 20), flags=flags)
-    a = scenes.lattice_block(6, 6, 6, d=0.012, origin=(0.52, 0.52, 0.52), seed=9)
+    a = scenes.lattice_block(7, 7, 7, d=0.01, origin=(0.52, 0.52, 0.52), seed=9)
     b = scenes.lattice_block(16, 16, 16, d=0.05, origin=(2.0, 0.2, 0.2), seed=9, stream=1)
     pos = np.concatenate([a, b], axis=1)
     vel = np.zeros_like(pos)
@@ -102,7 +102,7 @@ def test_tiles_with_long_lists_go_back_to_the_lanes(flags):
     o1 = oracle.step(sc.params, pos, vel, nsteps=1, diag="light")
     print(f"substep 1: tiles {st1['tiles']} of their particles {st1['tiled_particles']}, "
           f"max neighbors {int(o1['nbr_count'].max())}", flush=True)
-    assert int(o1["nbr_count"][: a.shape[1]].max()) > 128
+    assert int(o1["nbr_count"][: a.shape[1]].max()) > 256
     assert st1["tiled_particles"] == b.shape[1]
     assert np.array_equal(d1["nbr_count"], o1["nbr_count"])
     assert same_bits(out.pos, o["pos"])
