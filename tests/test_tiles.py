@@ -1,10 +1,12 @@
 """The LDS-tiled passes of sparse bricks (pbf_tiles.hip) against the CPU
 oracle and against the per-lane passes, bitwise.
 
-A brick of 4x4x4 cells runs through LDS when none of its cells holds 32
-particles or more, no particle has 256 candidates or more, and its 6^3-cell
-halo holds at most 2,048 particles; the others take the per-lane passes in
-the same substep.  Every sum visits the neighbors in the reference's order
+A brick of 4x4x4 cells runs through LDS when every cell of it has fewer than
+512 candidates (particles in its 27 cells), its 6^3-cell halo holds at most
+2,048 particles and no particle's list outgrows the tile list (256 entries
+below 2^18 particles, 128 from there on, where a brick also needs 32 particles
+and no coincident pair); the others take the per-lane passes in the same
+substep.  Every sum visits the neighbors in the reference's order
 (core.cpp:205-247), so the mixed substep must equal the oracle bit for bit."""
 import numpy as np
 import pytest
