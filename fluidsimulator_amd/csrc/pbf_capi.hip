@@ -127,6 +127,15 @@ int validate(int n_scenes, const fluid_hip_params* params) {
   return FLUID_HIP_OK;
 }
 
+// The tile passes evaluate the spiky factor of a pair with r2 < h2 as the
+// correctly rounded square root of max(r2, min_r2) without sqrtf's range
+// handling and without spiky's r > h test; both are exact when the largest
+// such argument rounds to a root <= h and the smallest is >= 2^-96.
+bool tile_terms_exact(float h, float h2, float min_r2) {
+  const float top = std::fmax(std::nextafter(h2, 0.0f), min_r2);
+  return min_r2 >= 0x1p-96f && std::sqrt(top) <= h;
+}
+
 // The loop invariants of core/src/core.cpp:138-148 and :270-274, in the
 // same float expressions, for the scenes of `params` (n_scenes entries).
 void compute_consts_of(Ctx& c, const fluid_hip_params* params, int n_scenes) {
@@ -188,7 +197,10 @@ void compute_consts_of(Ctx& c, const fluid_hip_params* params, int n_scenes) {
   // tile mode 0 (default) tiles from kTileAutoMinParticles particles: below
   // that a substep is bound by launches, to which the tile kernels add
   const bool tile_mode = c.tile_gather == 1 || (c.tile_gather == 0 && c.n >= fluidhip::kTileAutoMinParticles);
-  k.tiles = (k.runs && c.brick_shift == 2 && tile_mode && p.solver_iterations > 0) ? 1 : 0;
+  k.tiles = (k.runs && c.brick_shift == 2 && tile_mode && p.solver_iterations > 0 &&
+             tile_terms_exact(h, h2, k.min_r2))
+                ? 1
+                : 0;
   c.consts = k;
   c.iterations = p.solver_iterations;
   c.xsph = p.enable_xsph && p.visc_c != 0.0f;           // core.cpp:423

@@ -15,11 +15,18 @@ namespace fluidhip {
 
 // Brick tiles of the LDS-tiled gather passes (pbf_tiles.hip).
 constexpr int kTileMaxHalo = 2048;  // particles of a tile's 6^3-cell halo held in LDS
-// neighbors of a tile particle, at most (else its tile goes per-lane): in
-// batches under kForkMinParticles particles, which tile their piles of
-// coincident particles too, kTileListSmall
-constexpr int kTileList = 128;
+// A brick with a cell of this many candidates (particles in its 27 cells) or
+// more is no tile.
+constexpr int kTileMaxCand = 512;
+// neighbors of a tile particle, at most (else its tile goes per-lane).  In
+// batches of kForkMinParticles particles or more a list holds every neighbor
+// a tile particle can have (fewer than kTileMaxCand), 1 KiB per particle, so
+// no tile goes back and the lane map is built before the tile lists; smaller
+// batches, which tile their piles of coincident particles too, keep
+// kTileListSmall entries and hand back the tiles that overflow.
+constexpr int kTileList = 512;
 constexpr int kTileListSmall = 256;
+static_assert(kTileList >= kTileMaxCand - 1, "large-batch tiles must never overflow");
 constexpr int kForkMinParticles = 1 << 18;  // tile kernels on the side stream from this size
 constexpr int kTileAutoMinParticles = 1 << 15;  // default tile mode: tiles from this size
 constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
@@ -275,7 +282,7 @@ hipError_t enqueue_init_grid(Ctx& c);
 // The tile kernels (pbf_tiles.hip).
 hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k);
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
-                                  const float4* p0, bool store);
+                                  const float4* p0, bool store, hipStream_t st);
 hipError_t enqueue_tile_lambda(Ctx& c, const StepConsts& k, const TileArgs& ta, float4* P,
                                int write_rho, float4* sq, hipStream_t st);
 hipError_t enqueue_tile_delta(Ctx& c, const StepConsts& k, const TileArgs& ta, const float4* P,

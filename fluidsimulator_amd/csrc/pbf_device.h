@@ -97,16 +97,19 @@ __device__ __forceinline__ float poly6(float r2, const StepConsts& c) {
 // there, so both give the same bits; other x go through sqrtf itself.  The
 // gather passes call it once per neighbor, where those parts cost a third of
 // the square root.
+__device__ __forceinline__ float sqrt_core(float x) {
+  // the correction alone: finite x >= 2^-96 only
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __int_as_float(__float_as_int(s) - 1);
+  const float up = __int_as_float(__float_as_int(s) + 1);
+  const float rdn = fmaf(-dn, s, x);
+  const float rup = fmaf(-up, s, x);
+  const float r = rdn <= 0.0f ? dn : s;
+  return rup > 0.0f ? up : r;
+}
+
 __device__ __forceinline__ float sqrt_rn(float x) {
-  if (x >= 0x1p-96f && x <= 0x1.fffffep+127f) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float dn = __int_as_float(__float_as_int(s) - 1);
-    const float up = __int_as_float(__float_as_int(s) + 1);
-    const float rdn = fmaf(-dn, s, x);
-    const float rup = fmaf(-up, s, x);
-    const float r = rdn <= 0.0f ? dn : s;
-    return rup > 0.0f ? up : r;
-  }
+  if (x >= 0x1p-96f && x <= 0x1.fffffep+127f) return sqrt_core(x);
   return sqrtf(x);
 }
 

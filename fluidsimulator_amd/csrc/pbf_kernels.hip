@@ -2368,13 +2368,42 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                  c.posA, c.velA, c.posB, c.velB, PK(0), c.cellc, c.slot_cell,
                                  c.slot_key, c.flags);
   }));
-  // tiles and their neighbor lists come before the lane map: a tile with a
-  // particle of more neighbors than its list holds is handed back to the lanes
+  // Each pass's tile kernel runs on the side stream beside the per-lane
+  // kernel and its run fill on the main stream: they write disjoint slots
+  // (tile bricks hold no run of coincident particles, and the lane map leaves
+  // tile slots out) and read only what earlier passes finished.  fork() makes
+  // the side stream wait for the main stream's work so far, join() the main
+  // stream for the side stream's.  Both are edges of a captured graph too.
+  // An event record and a cross-stream wait leave the device idle for tens
+  // of microseconds (about 30 us per fork measured at 65,536 particles), so
+  // small batches, whose tile kernels take about as long, stay on one stream.
+  const bool forked = tiles && c.side != nullptr && c.n >= kForkMinParticles;
+  hipStream_t tst = forked ? c.side : st;
+  hipError_t fj = hipSuccess;
+  auto fork = [&] {
+    if (!forked || fj != hipSuccess) return;
+    fj = hipEventRecord(c.ev_fork, st);
+    if (fj == hipSuccess) fj = hipStreamWaitEvent(c.side, c.ev_fork, 0);
+  };
+  auto join = [&] {
+    if (!forked || fj != hipSuccess) return;
+    fj = hipEventRecord(c.ev_join, c.side);
+    if (fj == hipSuccess) fj = hipStreamWaitEvent(st, c.ev_join, 0);
+  };
+  // The tile lists come before the lane map when a tile can go back to the
+  // lanes (a particle with more neighbors than its list holds).  A large
+  // batch's lists hold every possible neighbor, so its tiles are final after
+  // classify: the tile lists are built on the side stream while the main
+  // stream builds the lane map and the lanes' lists (which write only
+  // non-tile slots: every tile slot heads its own run).
+  const bool early_lanes = forked && c.tile_list_cap >= kTileMaxCand - 1;
   if (tiles)
     FH_TRY(L("tiles", [&] {
       (void)enqueue_tile_classify(c, k);
-      (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr);
+      if (early_lanes) fork();
+      (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr, early_lanes ? tst : st);
     }));
+  FH_TRY(fj);
   if (runs)
     FH_TRY(L("runs", [&] {
       // runs and non-empty cells: flags, their scans (one dual scan), the
@@ -2406,30 +2435,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   FH_TRY(L("neighbors", [&] {
     k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
+    if (early_lanes) join();
   }));
-
-  // Each pass's tile kernel runs on the side stream beside the per-lane
-  // kernel and its run fill on the main stream: they write disjoint slots
-  // (tile bricks hold no run of coincident particles, and the lane map leaves
-  // tile slots out) and read only what earlier passes finished.  fork() makes
-  // the side stream wait for the main stream's work so far, join() the main
-  // stream for the side stream's.  Both are edges of a captured graph too.
-  // An event record and a cross-stream wait leave the device idle for tens
-  // of microseconds (about 30 us per fork measured at 65,536 particles), so
-  // small batches, whose tile kernels take about as long, stay on one stream.
-  const bool forked = tiles && c.side != nullptr && c.n >= kForkMinParticles;
-  hipStream_t tst = forked ? c.side : st;
-  hipError_t fj = hipSuccess;
-  auto fork = [&] {
-    if (!forked || fj != hipSuccess) return;
-    fj = hipEventRecord(c.ev_fork, st);
-    if (fj == hipSuccess) fj = hipStreamWaitEvent(c.side, c.ev_fork, 0);
-  };
-  auto join = [&] {
-    if (!forked || fj != hipSuccess) return;
-    fj = hipEventRecord(c.ev_join, c.side);
-    if (fj == hipSuccess) fj = hipStreamWaitEvent(st, c.ev_join, 0);
-  };
+  FH_TRY(fj);
 
   // velocity buffer that the commit writes: XSPH needs a second buffer to
   // stay Jacobi, so the commit goes to velB and XSPH back to velA.

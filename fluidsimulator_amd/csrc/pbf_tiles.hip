@@ -52,9 +52,6 @@ constexpr int kHaloCells = 216;
 constexpr int kThreads = 256;      // workgroup of the classify kernel
 constexpr int kPass = 512;         // workgroup of the neighbor and gather kernels
 constexpr int kPerThread = kTileMaxHalo / kPass;  // halo records staged per thread
-// Bricks with a cell of this many candidates (27-cell particles) or more take
-// the per-lane passes: the tile neighbor build tests every candidate.
-constexpr int kTileMaxCand = 512;
 // In batches of kForkMinParticles particles or more, bricks with fewer
 // particles take the per-lane passes: a tile's fixed cost (layout, halo
 // staging, a 512-thread workgroup) is not worth it for a few particles of
@@ -69,6 +66,38 @@ __device__ __forceinline__ bool tile_fatal(const unsigned* flags) {
 __device__ __forceinline__ bool coincide(const float3& a, const float3& b) {
   return ((__float_as_uint(a.x) ^ __float_as_uint(b.x)) | (__float_as_uint(a.y) ^ __float_as_uint(b.y)) |
           (__float_as_uint(a.z) ^ __float_as_uint(b.z))) == 0u;
+}
+
+// Pair terms of the gather passes for a pair with r2 < h2.  Tiles run only
+// when min_r2 >= 2^-96 and sqrt(max(pred(h2), min_r2)) <= h (tile_terms_exact
+// in pbf_capi.hip), so for such a pair the clamped argument of the square
+// root is a finite float >= 2^-96, where sqrt_core is the correctly rounded
+// square root, and r <= h, where spiky's r > h test is false: tile_poly6 and
+// tile_spiky return what poly6 and pair_spiky return (core.cpp:35-57 and the
+// clamp of core.cpp:303), bit for bit, without their range tests.
+__device__ __forceinline__ float tile_poly6(float r2, const StepConsts& c) {
+  const float diff = c.h2 - r2;
+  const float diff3 = diff * diff * diff;
+  return c.w_coeff * diff3;
+}
+
+__device__ __forceinline__ float tile_spiky(float r2, const StepConsts& c) {
+  // r2 < min_r2 ? min_r2 : r2 for 0 <= r2 < h2 and min_r2 > 0, where the
+  // bit patterns order as the floats do (one integer max)
+  const float r = sqrt_core(__int_as_float(max(__float_as_int(r2), __float_as_int(c.min_r2))));
+  const float diff = c.h - r;
+  return c.s_coeff * diff * diff;
+}
+
+// Marks the LDS reads of a visit as needed before its r2 < h2 test, so that
+// they are issued together (one LDS round trip per visit) instead of the
+// record's other fields being read after the test.  Emits no instruction.
+__device__ __forceinline__ void lds_all() {}
+
+template <typename... T>
+__device__ __forceinline__ void lds_all(float v, T... rest) {
+  asm volatile("" ::"v"(v));
+  lds_all(rest...);
 }
 
 // Scene of the brick whose first cell has key `k0` (scenes own whole bricks).
@@ -555,17 +584,20 @@ __global__ __launch_bounds__(kPass) void k_tile_lambda(StepConsts c, TileArgs ta
           const float dy = p.y - Y[r];
           const float dz = p.z - Z[r];
           const float r2 = dx * dx + dy * dy + dz * dz;
-          const bool same = (dx == 0.0f) & (dy == 0.0f) & (dz == 0.0f);
-          const bool grad = !same & (r2 < c.h2);
-          rho += same ? c.w0 : poly6(r2, c);
-          if (grad) {
-            const float f = pair_spiky(r2, c);
+          // A coincident neighbor (r2 = 0) adds gradient terms of +-0, which
+          // leave the sums as they are: none of them is ever -0 (each starts
+          // at +0 and a rounded sum is -0 only of two -0).
+          if (r2 < c.h2) {
+            rho += tile_poly6(r2, c);
+            const float f = tile_spiky(r2, c);
             const float gx = f * dx, gy = f * dy, gz = f * dz;
             const float jx = ngs * gx, jy = ngs * gy, jz = ngs * gz;
             gsx += gx;
             gsy += gy;
             gsz += gz;
             sg2 += jx * jx + jy * jy + jz * jz;
+          } else {
+            rho += r2 > c.h2 ? 0.0f : tile_poly6(r2, c);  // poly6: +0, or NaN
           }
         });
         rho += c.w0;
@@ -608,21 +640,21 @@ __global__ __launch_bounds__(kPass) void k_tile_delta(
       [&](int s, const TileInfo& T, const Nbrs& nb, const float4& pi) {
         float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float dx = pi.x - X[r];
-          const float dy = pi.y - Y[r];
-          const float dz = pi.z - Z[r];
+          const float x = X[r], y = Y[r], z = Z[r], w = W[r];
+          lds_all(x, y, z, w);
+          const float dx = pi.x - x;
+          const float dy = pi.y - y;
+          const float dz = pi.z - z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
-          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-          const float f = same ? c.f_min : pair_spiky(r2, c);
-          float sv = pi.w + W[r];
+          const float f = tile_spiky(r2, c);
+          float sv = pi.w + w;
           if (c.scorr_on) {
-            const float Wr = same ? c.w0 : poly6(r2, c);
+            const float Wr = tile_poly6(r2, c);
             const float ratio = Wr * c.scorr_inv_wdq;
             sv += c.scorr_neg_k * pow_ratio(ratio, c.scorr_n);
           }
           const float sf = sv * f;
-          if (same && isfinite(sf)) return;  // (sv f) * (+-0): +-0
           ddx += sf * dx;
           ddy += sf * dy;
           ddz += sf * dz;
@@ -694,15 +726,18 @@ __global__ __launch_bounds__(kPass) void k_tile_xsph(
         const float4 vi = me.v;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float dx = xi.x - X[r];
-          const float dy = xi.y - Y[r];
-          const float dz = xi.z - Z[r];
+          const float x = X[r], y = Y[r], z = Z[r];
+          const float u = U[r], v = V[r], q = Q[r], m = M[r];
+          lds_all(x, y, z, u, v, q, m);
+          const float dx = xi.x - x;
+          const float dy = xi.y - y;
+          const float dz = xi.z - z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
-          const float Wr = poly6(r2, c);
-          ax += (U[r] - vi.x) * Wr * M[r];
-          ay += (V[r] - vi.y) * Wr * M[r];
-          az += (Q[r] - vi.z) * Wr * M[r];
+          const float Wr = tile_poly6(r2, c);
+          ax += (u - vi.x) * Wr * m;
+          ay += (v - vi.y) * Wr * m;
+          az += (q - vi.z) * Wr * m;
         });
         float vx = vi.x + c.visc_c * ax;
         float vy = vi.y + c.visc_c * ay;
@@ -745,16 +780,16 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_omega(
         const float3 xi = me.x, vi = me.v;
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float dx = xi.x - X[r];
-          const float dy = xi.y - Y[r];
-          const float dz = xi.z - Z[r];
+          const float x = X[r], y = Y[r], z = Z[r], u = U[r], v = V[r], q = Q[r];
+          lds_all(x, y, z, u, v, q);
+          const float dx = xi.x - x;
+          const float dy = xi.y - y;
+          const float dz = xi.z - z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
-          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-          const float f = pair_spiky(r2, c);
+          const float f = tile_spiky(r2, c);
           const float gx = f * dx, gy = f * dy, gz = f * dz;
-          const float dvx = U[r] - vi.x, dvy = V[r] - vi.y, dvz = Q[r] - vi.z;
-          if (same && isfinite(dvx) && isfinite(dvy) && isfinite(dvz)) return;  // signed zero
+          const float dvx = u - vi.x, dvy = v - vi.y, dvz = q - vi.z;
           ox += dvy * gz - dvz * gy;
           oy += dvz * gx - dvx * gz;
           oz += dvx * gy - dvy * gx;
@@ -795,16 +830,16 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_apply(
         const float4 om = me.om;
         float ex = 0.0f, ey = 0.0f, ez = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float dx = xi.x - X[r];
-          const float dy = xi.y - Y[r];
-          const float dz = xi.z - Z[r];
+          const float x = X[r], y = Y[r], z = Z[r], o = O[r];
+          lds_all(x, y, z, o);
+          const float dx = xi.x - x;
+          const float dy = xi.y - y;
+          const float dz = xi.z - z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
-          const bool same = (dx == 0.0f && dy == 0.0f && dz == 0.0f);
-          const float f = pair_spiky(r2, c);
+          const float f = tile_spiky(r2, c);
           const float gx = f * dx, gy = f * dy, gz = f * dz;
-          const float co = O[r] - om.w;
-          if (same && isfinite(co)) return;  // co * (+-0): a signed zero
+          const float co = o - om.w;
           ex += co * gx;
           ey += co * gy;
           ez += co * gz;
@@ -851,8 +886,8 @@ hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k) {
 }
 
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
-                                  const float4* p0, bool store) {
-  k_tile_neighbors<<<kTileGrid, kPass, 0, c.stream>>>(
+                                  const float4* p0, bool store, hipStream_t st) {
+  k_tile_neighbors<<<kTileGrid, kPass, 0, st>>>(
       k, ta, p0, c.tile_rec, c.in_tile, c.ncount, c.noff, c.nwords, store ? c.nbr : nullptr,
       c.nbr_cap, c.nbr_total, c.flags);
   return hipGetLastError();

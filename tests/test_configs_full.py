@@ -3,7 +3,8 @@ CPU oracle (the restatement of core/src/core.cpp:119-615, pinned by the
 reference's own outputs in test_oracle.py).
 
 * 1,048,576-particle block (128x128x64, all flags): one substep from t=0
-  with grid, order, neighbor lists, rho and lambda bit-exact; one substep
+  with grid, order, neighbor lists, rho and lambda bit-exact; substeps 1-3
+  through the LDS tiles (eager and hipGraph), bitwise; one substep
   from the piled-up state after 30 substeps, bitwise, with the density
   statistics of that substep.
 * 256-scene ensemble (dims uniform in 8..32, 2.08M particles): 5 substeps
@@ -88,6 +89,27 @@ def test_million_block_first_substep_bit_exact():
     assert same_bits(r.vel, o["vel"])
     assert same_bits(d["rho"], o["rho"])
     assert same_bits(d["lambda"], o["lambda"])
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_million_block_early_substeps_through_tiles_bitwise(graph):
+    """Substeps 1-3 from t=0 with the default tile mode: nearly every
+    particle in an LDS tile, the tile kernels on the side stream and the
+    tile lists built beside the lane map."""
+    sc = scenes.million_block()
+    n = sc.state.n
+    steps = 3
+    with hip.HipSolver(sc.params, [n]) as s:
+        s.upload(sc.state.pos, sc.state.vel)
+        s.step(steps, graph=graph)
+        r = s.download()
+        st = s.stats()
+    print(f"million block, {steps} substeps (graph {graph}): {st['tiles']} tiles, "
+          f"{st['tiled_particles']} tiled particles", flush=True)
+    assert st["tiled_particles"] > 0.9 * n
+    o = oracle.step(sc.params, sc.state.pos, sc.state.vel, nsteps=steps)
+    assert same_bits(r.pos, o["pos"])
+    assert same_bits(r.vel, o["vel"])
 
 
 def test_million_block_piled_up_substep_bitwise():
