@@ -47,7 +47,9 @@ FLOPS_PER_PAIR = {"lambda": 33.0, "delta": 30.0, "xsph": 25.0, "vorticity_omega"
                   "vorticity_apply": 23.0}
 FP32_PEAK_TFLOPS = 157.3
 KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "tiles", "runs",
-           "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply"]
+           "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply",
+           "tile_neighbors", "tile_lambda", "tile_delta", "tile_xsph", "tile_vorticity_omega",
+           "tile_vorticity_apply"]
 
 
 def sync_all():
@@ -166,6 +168,21 @@ ROCPROF_NAME = {"predict_bbox": "k_predict_bbox", "lambda": "k_lambda", "delta":
                 "vorticity_apply": "k_vort_apply"}
 
 
+def window_bounds(pos: list, take: int) -> tuple:
+    """Bounds of the timed window on the ascending dispatch positions (start
+    times or dispatch ids) of a per-lane pass kernel: halfway to its launch
+    before the window and halfway past its last.  The pass's tile kernel is
+    launched next to each per-lane launch, so exactly its launches of the
+    window fall inside; those of another context (the early-regime record)
+    or of the warm-up fall outside."""
+    w = pos[-take:]
+    if not w:
+        return float("-inf"), float("-inf")
+    lo = (pos[-take - 1] + w[0]) / 2 if len(pos) > take else float("-inf")
+    hi = w[-1] + (w[-1] - w[-2]) / 2 if len(w) > 1 else float("inf")
+    return lo, hi
+
+
 def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | None = None,
                       iterations: int = 4) -> str:
     """HBM bytes per launch and L2 hit rate of the roofline kernels from
@@ -230,18 +247,22 @@ def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | No
 
     def window(d, key, per_step):
         """Per-launch values of pass `key` over the timed window: the per-lane
-        kernel and, where the pass has one, its LDS-tile kernel (both run in
-        every launch of the pass) summed launch by launch."""
+        kernel and, when the window's launches have one, its LDS-tile kernel,
+        summed launch by launch."""
         if d is None:
             return []
-        out = None
-        for name in (ROCPROF_NAME[key], "k_tile_" + ROCPROF_NAME[key][2:]):
-            vals = [v for _, v in sorted(x for k in d if name in k for x in d[k])]
-            vals = vals[-per_step * steps:]
-            if not vals:
-                continue
-            out = vals if out is None else [a + b for a, b in zip(out, vals)]
-        return out or []
+        take = per_step * steps
+        lane = sorted(x for k in d if ROCPROF_NAME[key] in k for x in d[k])
+        out = [v for _, v in lane[-take:]]
+        lo, hi = window_bounds([i for i, _ in lane], take)
+        tname = "k_tile_" + ROCPROF_NAME[key][2:]
+        tile = sorted(x for k in d if tname in k for x in d[k] if lo < x[0] < hi)
+        if tile:
+            if len(tile) != len(out):
+                raise SystemExit(f"{key}: {len(tile)} tile launches against {len(out)} "
+                                 "per-lane launches in the timed window")
+            out = [a + v for a, (_, v) in zip(out, tile)]
+        return out
 
     res = {"workload": f"million_block {dims[0]}x{dims[1]}x{dims[2]}, the timed window of "
                        "the bench command",
@@ -305,16 +326,43 @@ def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
               "(dispatches that returned at once, under 1% of the kernel's median, are",
               "left out: after a capacity overflow the substeps queued behind the failed",
               "one do no work, and the failed one runs again):"]
+    # a tile kernel counts only inside its per-lane pass kernel's window (the
+    # tile lists and classification inside predict_bbox's, one per substep);
+    # per-lane keys come first
+    bounds = {}
     for key, per_step in launches_per_step.items():
-        d = [r[0] for r in cur.execute(
-            "select duration from kernels where name like ? order by start", (f"%k_{key}%",))]
+        rows = cur.execute("select start, duration from kernels where name like ? order by start",
+                           (f"%k_{key}%",)).fetchall()
         take = steps * per_step
-        if d and take:
-            med = sorted(d)[len(d) // 2]
-            live = [x for x in d if x >= 0.01 * med]
-            w = live[-take:]
-            lines.append(f"  {key:18s} launches {len(w):6d}  avg_us {sum(w) / len(w) / 1e3:12.1f}"
-                         f"  (skipped {len(d) - len(live)})")
+        if not rows or not take:
+            continue
+        med = sorted(r[1] for r in rows)[len(rows) // 2]
+        live = [r for r in rows if r[1] >= 0.01 * med]
+        if key.startswith("tile_"):
+            lane = "predict_bbox" if key in ("tile_classify", "tile_neighbors") else key[5:]
+            lo, hi = bounds.get(lane, (float("-inf"), float("inf")))
+            w = [r[1] for r in live if lo < r[0] < hi]
+        else:
+            bounds[key] = window_bounds([r[0] for r in live], take)
+            w = [r[1] for r in live[-take:]]
+        if not w:
+            continue
+        lines.append(f"  {key:18s} launches {len(w):6d}  avg_us {sum(w) / len(w) / 1e3:12.1f}"
+                     f"  (skipped {len(rows) - len(live)})")
+    # the line's roofline_early: substeps 2-3 of the first context (the
+    # dispatches between its second and fourth predict_bbox)
+    pb = [r[0] for r in cur.execute("select start from kernels where name like "
+                                    "'%k_predict_bbox%' order by start")]
+    if len(pb) >= 4:
+        lines += ["", "early-regime record (the line's roofline_early): substeps 2-3 of the",
+                  "first context, from t = 0:"]
+        for key in launches_per_step:
+            w = [r[0] for r in cur.execute(
+                "select duration from kernels where name like ? and start >= ? and start < ?",
+                (f"%k_{key}%", pb[1], pb[3]))]
+            if w:
+                lines.append(f"  {key:18s} launches {len(w):6d}  avg_us "
+                             f"{sum(w) / len(w) / 1e3:12.1f}")
     return "\n".join(lines)
 
 
@@ -555,26 +603,38 @@ def read_kernels(s) -> dict:
     return out
 
 
-def roofline_record(kernels: dict, n: int, mean_neighbors: float) -> dict:
+def roofline_record(kernels: dict, n: int, mean_neighbors: float, n_tiled: int = 0) -> dict:
     """Roofline of the kernel with the most device time among the gather
-    passes: algorithmic bytes per launch (KERNEL_BYTES x n) over its average
-    launch time, against the HBM peak; the FP32 fraction from the algorithmic
-    flops per neighbor pair (FLOPS_PER_PAIR x mean neighbors x n).  Adds
-    both figures to every kernel's entry."""
+    passes: algorithmic bytes per launch (KERNEL_BYTES x the particles it
+    serves) over its average launch time, against the HBM peak; the FP32
+    fraction from the algorithmic flops per neighbor pair (FLOPS_PER_PAIR x
+    mean neighbors x particles).  A tile kernel ("tile_<pass>") serves the
+    n_tiled particles of the LDS tiles; with n_tiled > 0 the dominant kernel
+    is taken among the tile kernels.  Adds both figures to every kernel's
+    entry."""
+    def base(k):
+        return k[5:] if k.startswith("tile_") else k
+
+    def served(k):
+        return n_tiled if k.startswith("tile_") else n
+
     for k, v in kernels.items():
-        if k in KERNEL_BYTES:
+        b = base(k)
+        if b in KERNEL_BYTES and served(k) > 0:
             t = v["avg_us"] * 1e-6
-            v["algorithmic_GBs"] = KERNEL_BYTES[k] * n / t / 1e9
+            v["algorithmic_GBs"] = KERNEL_BYTES[b] * served(k) / t / 1e9
             v["hbm_frac"] = v["algorithmic_GBs"] / HBM_PEAK_GBS
-            if k in FLOPS_PER_PAIR:
-                v["algorithmic_TFLOPs"] = FLOPS_PER_PAIR[k] * mean_neighbors * n / t / 1e12
+            if b in FLOPS_PER_PAIR:
+                v["algorithmic_TFLOPs"] = FLOPS_PER_PAIR[b] * mean_neighbors * served(k) / t / 1e12
                 v["fp32_frac"] = v["algorithmic_TFLOPs"] / FP32_PEAK_TFLOPS
-    ranked = sorted((k for k in kernels if k in KERNEL_BYTES), key=lambda k: -kernels[k]["total_ms"])
-    dom = ranked[0]
+    pool = [k for k in kernels if "algorithmic_GBs" in kernels[k]
+            and k.startswith("tile_") == (n_tiled > 0)]
+    dom = max(pool, key=lambda k: kernels[k]["total_ms"])
     v = kernels[dom]
     return {"bound": "hbm", "kernel": dom, "achieved": v["algorithmic_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": v["hbm_frac"], "traffic": None,
-            "bytes_per_launch": KERNEL_BYTES[dom] * n,
+            "bytes_per_launch": KERNEL_BYTES[base(dom)] * served(dom),
+            "avg_us": v["avg_us"],
             "fp32_TFLOPs": v.get("algorithmic_TFLOPs"), "fp32_peak_TFLOPs": FP32_PEAK_TFLOPS,
             "fp32_frac": v.get("fp32_frac")}
 
@@ -639,7 +699,8 @@ def main():
         return
     if args.summarize_rocprof:
         print(summarize_rocprof(args.summarize_rocprof, args.steps,
-                                {"lambda": 4, "delta": 4, "xsph": 1, "vort_omega": 1,
+                                {"predict_bbox": 1, "lambda": 4, "delta": 4, "xsph": 1,
+                                 "vort_omega": 1,
                                  "vort_apply": 1, "neighbors": 1, "reorder": 1,
                                  "tile_lambda": 4, "tile_delta": 4, "tile_xsph": 1,
                                  "tile_vort_omega": 1, "tile_vort_apply": 1,
@@ -674,7 +735,8 @@ def main():
         se.profile(False)
         st_e = se.stats()
         se.close()
-        early = roofline_record(early_k, n, mean_neighbors=st_e["pairs"] / n)
+        tiled = st_e["tiled_particles"] if "tile_lambda" in early_k else 0
+        early = roofline_record(early_k, n, mean_neighbors=st_e["pairs"] / n, n_tiled=tiled)
         early["substeps"] = "2-3"
         early["mean_neighbors"] = st_e["pairs"] / n
         early["tiled_particles"] = st_e["tiled_particles"]

@@ -2217,6 +2217,12 @@ struct Launcher {
   Ctx& c;
   template <typename F>
   hipError_t operator()(const char* name, F&& f) {
+    return on(c.stream, name, f);
+  }
+  // events on stream `st` (a tile kernel on the side stream: its pass's own
+  // events on the main stream span the fork to the join)
+  template <typename F>
+  hipError_t on(hipStream_t st, const char* name, F&& f) {
     if (c.prof.on) {
       hipEvent_t a, b;
       if (c.prof.pool.size() >= 2) {
@@ -2230,9 +2236,9 @@ struct Launcher {
         e = hipEventCreate(&b);
         if (e != hipSuccess) return e;
       }
-      (void)hipEventRecord(a, c.stream);
+      (void)hipEventRecord(a, st);
       f();
-      (void)hipEventRecord(b, c.stream);
+      (void)hipEventRecord(b, st);
       c.prof.pending.push_back({name, a, b, c.prof.step});
     } else {
       f();
@@ -2401,7 +2407,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("tiles", [&] {
       (void)enqueue_tile_classify(c, k);
       if (early_lanes) fork();
-      (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr, early_lanes ? tst : st);
+      (void)L.on(early_lanes ? tst : st, "tile_neighbors", [&] {
+        (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr, early_lanes ? tst : st);
+      });
     }));
   FH_TRY(fj);
   if (runs)
@@ -2455,7 +2463,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("lambda", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0, sq, tst);
+        (void)L.on(tst, "tile_lambda",
+                   [&] { (void)enqueue_tile_lambda(c, k, ta, P, last ? 1 : 0, sq, tst); });
       }
       k_lambda<<<nl, bs, 0, st>>>(k, P, nv, cv, c.rho, last ? 1 : 0, sq, c.flags);
       fill(P, nullptr);
@@ -2465,8 +2474,10 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("delta", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1, pm,
-                                 last && rest_in_delta ? rm : nullptr, tst);
+        (void)L.on(tst, "tile_delta", [&] {
+          (void)enqueue_tile_delta(c, k, ta, P, Pn, last, rest_in_delta, v1, pm,
+                                   last && rest_in_delta ? rm : nullptr, tst);
+        });
       }
       if (!last)
         k_delta<true, false, false><<<nl, bs, 0, st>>>(k, P, Pn, nv, cv,
@@ -2509,7 +2520,9 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("xsph", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_xsph(c, k, ta, rest, rest && rec ? rec->RM : nullptr, tst);
+        (void)L.on(tst, "tile_xsph", [&] {
+          (void)enqueue_tile_xsph(c, k, ta, rest, rest && rec ? rec->RM : nullptr, tst);
+        });
       }
       if (rest)
         k_xsph<true><<<nl, bs, 0, st>>>(k, c.posA, c.velB, c.velA, nv, cv,
@@ -2526,7 +2539,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("vorticity_omega", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_vort_omega(c, k, ta, tst);
+        (void)L.on(tst, "tile_vorticity_omega",
+                   [&] { (void)enqueue_tile_vort_omega(c, k, ta, tst); });
       }
       k_vort_omega<<<nl, bs, 0, st>>>(k, c.posA, c.velA, c.omega, nv, cv, c.flags);
       fill(c.omega, nullptr);
@@ -2536,7 +2550,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     FH_TRY(L("vorticity_apply", [&] {
       if (tiles) {
         fork();
-        (void)enqueue_tile_vort_apply(c, k, ta, c.restitution_any, tst);
+        (void)L.on(tst, "tile_vorticity_apply",
+                   [&] { (void)enqueue_tile_vort_apply(c, k, ta, c.restitution_any, tst); });
       }
       if (c.restitution_any)
         k_vort_apply<true><<<nl, bs, 0, st>>>(k, c.posA, c.omega, c.velA, nv, cv,

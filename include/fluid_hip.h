@@ -308,7 +308,10 @@ int fluid_hip_step_noalloc(const fluid_hip_params* params, int64_t n, float* pos
 /* Per-kernel device time (HIP events on the context's stream) of the steps
  * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_count",
  * "scan", "sort", "reorder", "tiles", "runs", "neighbors", "lambda", "delta", "xsph",
- * "vorticity_omega", "vorticity_apply".  Launches of a substep that
+ * "vorticity_omega", "vorticity_apply"; with LDS tiles also "tile_neighbors",
+ * "tile_lambda", "tile_delta", "tile_xsph", "tile_vorticity_omega" and
+ * "tile_vorticity_apply", the tile kernels alone, timed on the stream they run
+ * on (a pass's own name spans its per-lane and tile kernels).  Launches of a substep that
  * overflowed a capacity, and of the substeps queued behind it, are not
  * counted (fluid_hip_step runs them again after growing the tables). */
 int fluid_hip_profile(fluid_hip_ctx* ctx, int32_t enable);

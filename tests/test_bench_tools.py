@@ -117,3 +117,68 @@ def test_more_than_one_gpu_is_refused_with_a_message():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_tile_launches_count_only_inside_the_window(tmp_path):
+    """An early-regime context runs 3 substeps with tile kernels before the
+    timed context; the timed window (per-lane kernels only, or per-lane and
+    tile kernels launched side by side) must not take the early tile launches
+    in, in the traffic summary or the kernel-trace window."""
+    copy_kib = 256 * 1024
+    _pmc_db(str(tmp_path / "pmc_cfetch" / "c_results.db"),
+            [(1, "__amd_rocclr_copyBuffer", "FETCH_SIZE", copy_kib)])
+    _pmc_db(str(tmp_path / "pmc_cwrite" / "c_results.db"),
+            [(1, "__amd_rocclr_copyBuffer", "WRITE_SIZE", copy_kib)])
+
+    def record(tiles_in_window):
+        fetch, kt, did = [], [], 0
+        # early context: 3 substeps, tile kernel (500 KiB) launched before
+        # the per-lane one (1 KiB)
+        for _ in range(3 * 4):
+            did += 1
+            fetch.append((did, "k_tile_lambda", "FETCH_SIZE", 500.0))
+            kt.append(("k_tile_lambda", did * 10, 50_000))
+            did += 1
+            fetch.append((did, "k_lambda", "FETCH_SIZE", 1.0))
+            kt.append(("k_lambda", did * 10, 10_000))
+            did += 1
+            fetch.append((did, "k_delta", "FETCH_SIZE", 1.0))
+            kt.append(("k_delta", did * 10, 10_000))
+        # timed context: 2 warm-up and 4 timed substeps of 100 KiB per-lane
+        # launches, with 7 KiB tile launches beside them when tiled (each
+        # lambda launch followed by a delta launch, as in a substep)
+        for step in range(6):
+            for _ in range(4):
+                if tiles_in_window:
+                    did += 1
+                    fetch.append((did, "k_tile_lambda", "FETCH_SIZE", 7.0))
+                    kt.append(("k_tile_lambda", did * 10, 70_000))
+                did += 1
+                fetch.append((did, "k_lambda", "FETCH_SIZE", 100.0))
+                kt.append(("k_lambda", did * 10, 1_000_000))
+                did += 1
+                fetch.append((did, "k_delta", "FETCH_SIZE", 100.0))
+                kt.append(("k_delta", did * 10, 1_000_000))
+        return fetch, kt
+
+    for tiles_in_window, expect_kib, tile_line in ((False, 100.0, None), (True, 107.0, 16)):
+        sub = tmp_path / f"t{int(tiles_in_window)}"
+        fetch, kt = record(tiles_in_window)
+        for d in ("pmc_cfetch", "pmc_cwrite"):
+            os.makedirs(sub / d, exist_ok=True)
+            os.link(tmp_path / d / "c_results.db", sub / d / "c_results.db")
+        _pmc_db(str(sub / "pmc_fetch" / "f_results.db"), fetch)
+        _pmc_db(str(sub / "pmc_write" / "w_results.db"),
+                [(i, n, "WRITE_SIZE", 0.0) for i, n, _, _ in fetch])
+        out = str(sub / "traffic.json")
+        bench.summarize_traffic(str(sub), out, steps=4)
+        r = json.load(open(out))["kernels"]["lambda"]
+        assert r["launches"] == 16
+        assert abs(r["fetch_bytes_per_launch"] - expect_kib * 1024) < 1e-6
+        _kernel_db(str(sub / "kt" / "k_results.db"), kt)
+        text = bench.summarize_rocprof(str(sub / "kt"), 4, {"lambda": 4, "tile_lambda": 4})
+        lines = [l for l in text.splitlines() if l.strip().startswith("tile_lambda")]
+        if tile_line is None:
+            assert not lines
+        else:
+            assert f"launches     {tile_line}" in lines[0] and "avg_us         70.0" in lines[0]
