@@ -300,7 +300,8 @@ def summarize_traffic(outdir: str, json_path: str | None = None, steps: int | No
     return "\n".join(lines)
 
 
-def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
+def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict,
+                      early: bool = True) -> str:
     """Text summary of a `rocprofv3 --kernel-trace --stats` run of this
     script (rocpd database output): per-kernel totals over every dispatch of
     the command, and per-kernel averages over the last `steps` substeps
@@ -353,7 +354,7 @@ def summarize_rocprof(outdir: str, steps: int, launches_per_step: dict) -> str:
     # dispatches between its second and fourth predict_bbox)
     pb = [r[0] for r in cur.execute("select start from kernels where name like "
                                     "'%k_predict_bbox%' order by start")]
-    if len(pb) >= 4:
+    if early and len(pb) >= 4:
         lines += ["", "early-regime record (the line's roofline_early): substeps 2-3 of the",
                   "first context, from t = 0:"]
         for key in launches_per_step:
@@ -406,6 +407,21 @@ def bench_ensemble(args) -> dict:
         s.close()
         return dt, out, kern, st
 
+    if args.batched_only:
+        # the batched context alone, eagerly, with per-kernel events: what a
+        # kernel trace of the ensemble records (no graph replays, no
+        # per-scene contexts)
+        t_eager, _, kern, st = run_batched(False, prof=not args.no_profile)
+        return {"metric": "ensemble substeps/s (all scenes per substep), batched eager only",
+                "value": args.steps / t_eager, "unit": "steps/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": 1000.0 * t_eager / args.steps, "higher_is_better": True,
+                "dtype": "f32", "data": "synthetic (seeded jittered lattices)",
+                "config": {"workload": f"ensemble {args.scenes} scenes, dims U{{8..32}}^3, "
+                                       "all flags, 4 iters, box 3^3",
+                           "particles": n, "scenes": args.scenes},
+                "tiles": tiles, "tiled_particles_last_step": st["tiled_particles"],
+                "kernels_eager": kern}
     t_graph, out_graph, _, st_graph = run_batched(True)
     t_eager, out_eager, _, _ = run_batched(False)
     _, _, kern, _ = run_batched(False, prof=not args.no_profile)
@@ -671,6 +687,8 @@ def main():
     ap.add_argument("--write-tuned", type=str, default="", metavar="JSON",
                     help="with --autotune: write the tuner's choice for this warm-up here")
     ap.add_argument("--scenes", type=int, default=256, help="ensemble size")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="ensemble: time the batched context eagerly and nothing else")
     ap.add_argument("--tiles", choices=["default", "auto", "on", "off"], default="default",
                     help="ensemble and adjoint workloads: the LDS-tiled passes (auto: the "
                          "library default, on from 32,768 particles with brick edge 4; "
@@ -704,7 +722,8 @@ def main():
                                  "vort_apply": 1, "neighbors": 1, "reorder": 1,
                                  "tile_lambda": 4, "tile_delta": 4, "tile_xsph": 1,
                                  "tile_vort_omega": 1, "tile_vort_apply": 1,
-                                 "tile_neighbors": 1, "tile_classify": 1}))
+                                 "tile_neighbors": 1, "tile_classify": 1},
+                                early=args.workload == "million_block"))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
