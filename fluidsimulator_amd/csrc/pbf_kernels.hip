@@ -244,11 +244,27 @@ __global__ __launch_bounds__(1024) void k_predict_bbox(
       b[0] = mn0, b[1] = mn1, b[2] = mn2, b[3] = mx0, b[4] = mx1, b[5] = mx2;
     }
   } else {
-    // a tile that straddles scenes: per-particle atomics (rare)
-    if (active) {
-      int* b = bbox + 6 * sc;
-      atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
-      atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+    // a tile that straddles scenes: a wavefront inside one scene reduces and
+    // takes one set of atomics, a wavefront that straddles two or more takes
+    // them per particle (in a batch of small scenes many tiles straddle, and
+    // per-particle atomics on one scene's six words serialize)
+    const int w0 = base + static_cast<int>(threadIdx.x & ~(kWave - 1));
+    const int wl = min(w0 + kWave, c.n) - 1;
+    if (w0 <= wl) {  // uniform per wavefront
+      const int sw = scene_of(w0, seg, c.n_scenes);
+      if (sw == scene_of(wl, seg, c.n_scenes)) {
+        mn0 = wave_min(mn0), mn1 = wave_min(mn1), mn2 = wave_min(mn2);
+        mx0 = wave_max(mx0), mx1 = wave_max(mx1), mx2 = wave_max(mx2);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+          int* b = bbox + 6 * sw;
+          atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
+          atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+        }
+      } else if (active) {
+        int* b = bbox + 6 * sc;
+        atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
+        atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+      }
     }
     if (threadIdx.x == 0) {
       int* b = tile_bbox + 6 * tile;
@@ -258,9 +274,10 @@ __global__ __launch_bounds__(1024) void k_predict_bbox(
   }
 }
 
-// One workgroup: per-scene table extents and key offsets; resets the
-// per-step counters.
-__global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __restrict__ seg,
+// One workgroup of kGridSetupThreads: per-scene table extents and key
+// offsets; resets the per-step counters.
+constexpr int kGridSetupThreads = 1024;
+__global__ __launch_bounds__(kGridSetupThreads) void k_grid_setup(int n_scenes, const int* __restrict__ seg,
                                                     const int* __restrict__ tile_bbox,
                                                     int tile_size, int* __restrict__ bbox,
                                                     GridDesc* __restrict__ grid,
@@ -271,7 +288,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(int n_scenes, const int* __r
                                                     unsigned long long* __restrict__ need_cells,
                                                     int* __restrict__ steps_started,
                                                     int* __restrict__ tile_count) {
-  __shared__ long long sh[257];
+  __shared__ long long sh[kGridSetupThreads];
   __shared__ long long carry;
   // After a failed substep the counters keep what the host needs to size
   // the retry; only the bounding boxes are reset.
@@ -2335,7 +2352,7 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                       c.tile_bbox, c.flags);
   }));
   FH_TRY(L("grid_setup", [&] {
-    k_grid_setup<<<1, 256, 0, st>>>(c.n_scenes, c.seg, c.tile_bbox, bs, c.bbox, c.grid,
+    k_grid_setup<<<1, kGridSetupThreads, 0, st>>>(c.n_scenes, c.seg, c.tile_bbox, bs, c.bbox, c.grid,
                                     c.total_cells, c.cell_cap, c.brick_shift,
                                     c.flags, c.nbr_total, c.need_cells, c.steps_started,
                                     tiles ? c.tile_count : nullptr);
