@@ -13,14 +13,19 @@
 //                 per-tile bounding boxes of cells        (core.cpp:150-161)
 //   grid_setup    per-scene table origin/extent, key offsets, counters
 //   cell_count    key of each particle, wavefront-aggregated cell histogram
-//   scan x3       exclusive scan of the histogram -> cell starts
+//   scan          exclusive scan of the histogram -> cell starts (two
+//                 launches: range sums, then each range scanned)
 //   sort          keys in original-id order, then 8-bit LSD radix passes
 //                 (digit histogram, its scan, stable ballot-ranked scatter):
 //                 sorting stably by key leaves each cell's particles in
 //                 ascending id, the reference's order (core.cpp:164-203)
 //   reorder       gather each particle's state into its slot, recompute pred0
+//   tiles         with 4-cell bricks: the bricks that run through LDS and
+//                 their neighbor lists (pbf_tiles.hip; on the side stream in
+//                 large batches)
 //   runs          runs of identical consecutive particles and the lane map of
-//                 the gather passes (dense cells from a wave boundary)
+//                 the gather passes (dense cells from a wave boundary; tile
+//                 slots get no lane)
 //   neighbors     27 cells in (dz, dy, dx) order, each by ascending id,
 //                 j != i and r^2 < h^2 on pred0       (core.cpp:205-247)
 //   lambda/delta  x solver_iterations, Jacobi double buffer (core.cpp:277-408);
@@ -29,6 +34,8 @@
 //   vorticity     omega, then eta + confinement        (core.cpp:468-571)
 //   plane restitution/friction fused into whichever kernel ends the chain
 //                                                      (core.cpp:573-612)
+// Each gather pass also runs its tile kernel, and after it k_run_fill copies
+// each run's outputs from its first slot to the others.
 
 #include <climits>
 
