@@ -295,6 +295,30 @@ def test_recorded_tiles_give_the_same_gradients(kind, restitution):
         assert s.stats()["tiles"] > 0
 
 
+def test_recorded_tiles_in_a_large_batch_give_the_same_gradients():
+    """From 2^18 particles the tile kernels run on the side stream and the
+    tile lists (stored as slot words for the reverse pass) are built beside
+    the lanes' lists, both appending to one list buffer: the gradients stay
+    bitwise those of recorded substeps without tiles."""
+    p, pos, vel, _ = make_scene("reference", (80, 58, 57))
+    n = pos.shape[1]
+    assert n >= 1 << 18
+    steps = 2
+    tgt = pos.astype(np.float64) + 0.01
+    off = hip_gradients(p, pos, vel, steps, tgt, every=steps, tile_gather=2)
+    for every in (steps, 1):
+        on = hip_gradients(p, pos, vel, steps, tgt, every=every, tile_gather=1)
+        assert on[0] == off[0]
+        for a, b in zip(on[1:], off[1:]):
+            assert np.array_equal(np.asarray(a).view(np.uint32), np.asarray(b).view(np.uint32)), every
+    with hip.HipSolver(p, [n], tile_gather=1) as s:
+        s.upload(pos, vel)
+        s.step(1)
+        st = s.stats()
+    print(f"{n} particles: {st['tiles']} tiles, {st['tiled_particles']} tiled", flush=True)
+    assert st["tiled_particles"] > n // 2
+
+
 def test_backward_repeatable_and_state_restored():
     p, pos, vel, vtg = make_scene("stated", (8, 8, 8))
     n = pos.shape[1]
