@@ -89,17 +89,6 @@ __device__ __forceinline__ float tile_spiky(float r2, const StepConsts& c) {
   return c.s_coeff * diff * diff;
 }
 
-// Marks the LDS reads of a visit as needed before its r2 < h2 test, so that
-// they are issued together (one LDS round trip per visit) instead of the
-// record's other fields being read after the test.  Emits no instruction.
-__device__ __forceinline__ void lds_all() {}
-
-template <typename... T>
-__device__ __forceinline__ void lds_all(float v, T... rest) {
-  asm volatile("" ::"v"(v));
-  lds_all(rest...);
-}
-
 // Scene of the brick whose first cell has key `k0` (scenes own whole bricks).
 __device__ __forceinline__ int scene_of_key(int k0, const GridDesc* __restrict__ grid,
                                             int n_scenes) {
@@ -640,15 +629,13 @@ __global__ __launch_bounds__(kPass) void k_tile_delta(
       [&](int s, const TileInfo& T, const Nbrs& nb, const float4& pi) {
         float ddx = 0.0f, ddy = 0.0f, ddz = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float x = X[r], y = Y[r], z = Z[r], w = W[r];
-          lds_all(x, y, z, w);
-          const float dx = pi.x - x;
-          const float dy = pi.y - y;
-          const float dz = pi.z - z;
+          const float dx = pi.x - X[r];
+          const float dy = pi.y - Y[r];
+          const float dz = pi.z - Z[r];
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
           const float f = tile_spiky(r2, c);
-          float sv = pi.w + w;
+          float sv = pi.w + W[r];
           if (c.scorr_on) {
             const float Wr = tile_poly6(r2, c);
             const float ratio = Wr * c.scorr_inv_wdq;
@@ -726,18 +713,15 @@ __global__ __launch_bounds__(kPass) void k_tile_xsph(
         const float4 vi = me.v;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float x = X[r], y = Y[r], z = Z[r];
-          const float u = U[r], v = V[r], q = Q[r], m = M[r];
-          lds_all(x, y, z, u, v, q, m);
-          const float dx = xi.x - x;
-          const float dy = xi.y - y;
-          const float dz = xi.z - z;
+          const float dx = xi.x - X[r];
+          const float dy = xi.y - Y[r];
+          const float dz = xi.z - Z[r];
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
           const float Wr = tile_poly6(r2, c);
-          ax += (u - vi.x) * Wr * m;
-          ay += (v - vi.y) * Wr * m;
-          az += (q - vi.z) * Wr * m;
+          ax += (U[r] - vi.x) * Wr * M[r];
+          ay += (V[r] - vi.y) * Wr * M[r];
+          az += (Q[r] - vi.z) * Wr * M[r];
         });
         float vx = vi.x + c.visc_c * ax;
         float vy = vi.y + c.visc_c * ay;
@@ -780,16 +764,14 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_omega(
         const float3 xi = me.x, vi = me.v;
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float x = X[r], y = Y[r], z = Z[r], u = U[r], v = V[r], q = Q[r];
-          lds_all(x, y, z, u, v, q);
-          const float dx = xi.x - x;
-          const float dy = xi.y - y;
-          const float dz = xi.z - z;
+          const float dx = xi.x - X[r];
+          const float dy = xi.y - Y[r];
+          const float dz = xi.z - Z[r];
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
           const float f = tile_spiky(r2, c);
           const float gx = f * dx, gy = f * dy, gz = f * dz;
-          const float dvx = u - vi.x, dvy = v - vi.y, dvz = q - vi.z;
+          const float dvx = U[r] - vi.x, dvy = V[r] - vi.y, dvz = Q[r] - vi.z;
           ox += dvy * gz - dvz * gy;
           oy += dvz * gx - dvx * gz;
           oz += dvx * gy - dvy * gx;
@@ -830,16 +812,14 @@ __global__ __launch_bounds__(kPass) void k_tile_vort_apply(
         const float4 om = me.om;
         float ex = 0.0f, ey = 0.0f, ez = 0.0f;
         tile_walk(ta.tile_list, c.n, s, nb.cnt, nb.first, [&](int r) {
-          const float x = X[r], y = Y[r], z = Z[r], o = O[r];
-          lds_all(x, y, z, o);
-          const float dx = xi.x - x;
-          const float dy = xi.y - y;
-          const float dz = xi.z - z;
+          const float dx = xi.x - X[r];
+          const float dy = xi.y - Y[r];
+          const float dz = xi.z - Z[r];
           const float r2 = dx * dx + dy * dy + dz * dz;
           if (!(r2 < c.h2)) return;
           const float f = tile_spiky(r2, c);
           const float gx = f * dx, gy = f * dy, gz = f * dz;
-          const float co = o - om.w;
+          const float co = O[r] - om.w;
           ex += co * gx;
           ey += co * gy;
           ez += co * gz;
