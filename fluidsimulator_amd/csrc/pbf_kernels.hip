@@ -2423,19 +2423,21 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   // The tile lists come before the lane map when a tile can go back to the
   // lanes (a particle with more neighbors than its list holds).  A large
   // batch's lists hold every possible neighbor, so its tiles are final after
-  // classify: the tile lists are built on the side stream while the main
-  // stream builds the lane map and the lanes' lists (which write only
-  // non-tile slots: every tile slot heads its own run).
+  // classify: the lane map is built first (its short kernels would wait for
+  // CUs behind the tile list build), then the tile lists are built on the
+  // side stream while the main stream builds the lanes' lists (which write
+  // only non-tile slots: every tile slot heads its own run).
   const bool early_lanes = forked && c.tile_list_cap >= kTileMaxCand - 1;
+  auto tile_lists = [&](hipStream_t s) {
+    (void)L.on(s, "tile_neighbors", [&] {
+      (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr, s);
+    });
+  };
   if (tiles)
     FH_TRY(L("tiles", [&] {
       (void)enqueue_tile_classify(c, k);
-      if (early_lanes) fork();
-      (void)L.on(early_lanes ? tst : st, "tile_neighbors", [&] {
-        (void)enqueue_tile_neighbors(c, k, ta, PK(0), rec != nullptr, early_lanes ? tst : st);
-      });
+      if (!early_lanes) tile_lists(st);
     }));
-  FH_TRY(fj);
   if (runs)
     FH_TRY(L("runs", [&] {
       // runs and non-empty cells: flags, their scans (one dual scan), the
@@ -2465,6 +2467,10 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
           tiles ? c.in_tile : nullptr, c.flags);
     }));
   FH_TRY(L("neighbors", [&] {
+    if (early_lanes) {
+      fork();
+      tile_lists(tst);
+    }
     k_neighbors<<<nl, bs, 0, st>>>(k, PK(0), cv, c.noff, c.nwords,
                                    c.ncount, c.nbr, c.nbr_cap, c.nbr_total, c.flags);
     if (early_lanes) join();

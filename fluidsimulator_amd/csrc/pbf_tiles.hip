@@ -471,15 +471,28 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
       for (int row = 0; row < 9; ++row) {
         const int hc = ((lz + row / 3) * kHalo + ly + row % 3) * kHalo + lx;
         const int lo = L.seg[hc], hi = L.seg[hc + 3];
-        for (int r = lo; r < hi; ++r) {
-          if (r == self || !near0(me, make_float3(X[r], Y[r], Z[r]), c)) continue;
-          w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-          w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-          w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-          w3 = (w3 >> 16) | (static_cast<unsigned>(r) << 16);
-          if ((k & 7) == 7 && k < ta.list_cap)
-            list[static_cast<size_t>(k >> 3) * c.n + s] = make_uint4(w0, w1, w2, w3);
-          ++k;
+        // 32 candidates at a time: first the neighbor tests as a bit mask,
+        // then the mask's bits in ascending order (the candidates' order);
+        // the insertion runs once per neighbor, not once per candidate
+        for (int b = lo; b < hi; b += 32) {
+          const int nb = min(32, hi - b);
+          unsigned m = 0u;
+          for (int j = 0; j < nb; ++j) {
+            const int r = b + j;
+            if (near0(me, make_float3(X[r], Y[r], Z[r]), c)) m |= 1u << j;
+          }
+          if (self >= b && self < b + 32) m &= ~(1u << (self - b));
+          while (m != 0u) {
+            const int r = b + __builtin_ctz(m);
+            m &= m - 1u;
+            w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+            w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+            w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+            w3 = (w3 >> 16) | (static_cast<unsigned>(r) << 16);
+            if ((k & 7) == 7 && k < ta.list_cap)
+              list[static_cast<size_t>(k >> 3) * c.n + s] = make_uint4(w0, w1, w2, w3);
+            ++k;
+          }
         }
       }
       if ((k & 7) != 0 && k < ta.list_cap) {
