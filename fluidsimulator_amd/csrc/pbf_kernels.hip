@@ -192,6 +192,24 @@ __global__ void k_init_bbox(int n_scenes, int* bbox) {
 
 // ---------------------------------------------------------------- grid build
 
+// Calls fn(leader, mine) once per distinct key among the lanes with
+// want == true: `leader` is the lowest such lane holding the key and `mine`
+// marks the lanes that share it.  Wavefront-uniform control flow; every
+// lane must be present.
+template <typename Fn>
+__device__ __forceinline__ void for_each_key_group(bool want, int key, Fn&& fn) {
+  bool pend = want;
+  while (true) {
+    const unsigned long long m = __ballot(pend);
+    if (m == 0ull) break;
+    const int leader = __builtin_ctzll(m);
+    const int kl = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = pend && key == kl;
+    fn(leader, mine);
+    pend = pend && !mine;
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_predict_bbox(
     StepConsts c, const float4* __restrict__ pos, const float4* __restrict__ vel,
     const ScenePlanes* __restrict__ sp, const int* __restrict__ seg,
@@ -251,28 +269,22 @@ __global__ __launch_bounds__(1024) void k_predict_bbox(
       b[0] = mn0, b[1] = mn1, b[2] = mn2, b[3] = mx0, b[4] = mx1, b[5] = mx2;
     }
   } else {
-    // a tile that straddles scenes: a wavefront inside one scene reduces and
-    // takes one set of atomics, a wavefront that straddles two or more takes
-    // them per particle (in a batch of small scenes many tiles straddle, and
-    // per-particle atomics on one scene's six words serialize)
-    const int w0 = base + static_cast<int>(threadIdx.x & ~(kWave - 1));
-    const int wl = min(w0 + kWave, c.n) - 1;
-    if (w0 <= wl) {  // uniform per wavefront
-      const int sw = scene_of(w0, seg, c.n_scenes);
-      if (sw == scene_of(wl, seg, c.n_scenes)) {
-        mn0 = wave_min(mn0), mn1 = wave_min(mn1), mn2 = wave_min(mn2);
-        mx0 = wave_max(mx0), mx1 = wave_max(mx1), mx2 = wave_max(mx2);
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-          int* b = bbox + 6 * sw;
-          atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
-          atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
-        }
-      } else if (active) {
+    // a tile that straddles scenes: each wavefront reduces the lanes of each
+    // of its scenes and takes one set of atomics per scene (in a batch of
+    // small scenes many tiles straddle, and per-particle atomics on one
+    // scene's six words serialize)
+    const int lane = threadIdx.x & (kWave - 1);
+    for_each_key_group(active, sc, [&](int leader, bool mine) {
+      const int a0 = wave_min(mine ? mn0 : INT_MAX), a1 = wave_min(mine ? mn1 : INT_MAX),
+                a2 = wave_min(mine ? mn2 : INT_MAX);
+      const int b0 = wave_max(mine ? mx0 : INT_MIN), b1 = wave_max(mine ? mx1 : INT_MIN),
+                b2 = wave_max(mine ? mx2 : INT_MIN);
+      if (lane == leader) {
         int* b = bbox + 6 * sc;
-        atomicMin(b + 0, mn0), atomicMin(b + 1, mn1), atomicMin(b + 2, mn2);
-        atomicMax(b + 3, mx0), atomicMax(b + 4, mx1), atomicMax(b + 5, mx2);
+        atomicMin(b + 0, a0), atomicMin(b + 1, a1), atomicMin(b + 2, a2);
+        atomicMax(b + 3, b0), atomicMax(b + 4, b1), atomicMax(b + 5, b2);
       }
-    }
+    });
     if (threadIdx.x == 0) {
       int* b = tile_bbox + 6 * tile;
       b[0] = b[1] = b[2] = INT_MAX;
@@ -409,23 +421,6 @@ constexpr int kCandPoints = 8;
 // 3 points 36.0, 4 points 35.7.
 constexpr int kTilePoints = 2;
 
-// Calls fn(leader, mine) once per distinct key among the lanes with
-// want == true: `leader` is the lowest such lane holding the key and `mine`
-// marks the lanes that share it.  Wavefront-uniform control flow; every
-// lane must be present.
-template <typename Fn>
-__device__ __forceinline__ void for_each_key_group(bool want, int key, Fn&& fn) {
-  bool pend = want;
-  while (true) {
-    const unsigned long long m = __ballot(pend);
-    if (m == 0ull) break;
-    const int leader = __builtin_ctzll(m);
-    const int kl = __builtin_amdgcn_readlane(key, leader);
-    const bool mine = pend && key == kl;
-    fn(leader, mine);
-    pend = pend && !mine;
-  }
-}
 
 __global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
                                                      const uint64_t* __restrict__ cellc,
