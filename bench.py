@@ -385,7 +385,7 @@ def bench_ensemble(args) -> dict:
 
     # the ensemble's default is tiles off: its scenes turn to sparse spray and
     # wall piles within a few substeps, where the per-lane passes are faster
-    # (240 substeps: 5.59 ms per substep batched, against 6.07 with tiles)
+    # (240 substeps: 5.50 ms per substep batched, against 6.07 with tiles)
     tiles = "off" if args.tiles == "default" else args.tiles
     tg = {"auto": 0, "on": 1, "off": 2}[tiles]
 
