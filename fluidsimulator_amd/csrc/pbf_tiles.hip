@@ -15,9 +15,11 @@
 //     halo cell (the cells' particle counts, scanned in halo order z, y, x)
 //     and each halo cell's first slot, and marks the brick's slots so that
 //     the lane map of the per-lane passes leaves them out;
-//   * neighbors (once per substep, before the lane map): stages the halo's
-//     p^0 in LDS and writes each particle's neighbor list as LDS records of
-//     the halo: the candidates are the 27 cells in (dz, dy, dx) order, each
+//   * neighbors (once per substep; in small batches before the lane map, in
+//     large ones, whose lists cannot overflow, beside the lanes' lists):
+//     stages the halo's p^0 in LDS and writes each particle's neighbor list
+//     as LDS records of the halo: the candidates are the 27 cells in
+//     (dz, dy, dx) order, each
 //     by ascending slot (= ascending id), and candidate j is a neighbor when
 //     it is not the particle itself and |p^0_i - p^0_j|^2 < h^2 (near0, the
 //     reference's list predicate, core.cpp:205-247).  A tile with a particle
