@@ -573,12 +573,20 @@ __global__ __launch_bounds__(1024) void k_sort_keys(StepConsts c, const float4* 
   vout[id] = p;
 }
 
+// Every key lies below the substep's table size, so a pass whose digits
+// start at or above its bit length has only digit 0: the stable pass keeps
+// the order, its histogram is not needed and its scatter is a copy.
+__device__ __forceinline__ bool pass_is_identity(const int* total_cells, int shift) {
+  return static_cast<long long>(*total_cells) <= (1ll << shift);
+}
+
 // hist[d * G + b]: items of workgroup b (kRadixTile of them) with digit d.
 __global__ __launch_bounds__(256) void k_radix_hist(int n, const int* __restrict__ kin, int shift,
                                                     int* __restrict__ hist,
+                                                    const int* __restrict__ total_cells,
                                                     const unsigned* __restrict__ flags) {
   __shared__ int h[256];
-  if (fatal(flags)) return;
+  if (fatal(flags) || pass_is_identity(total_cells, shift)) return;
   h[threadIdx.x] = 0;
   __syncthreads();
   const int b0 = blockIdx.x * kRadixTile;
@@ -597,11 +605,20 @@ __global__ __launch_bounds__(256) void k_radix_scatter(int n, const int* __restr
                                                        int* __restrict__ kout,
                                                        int* __restrict__ vout, int shift,
                                                        const int* __restrict__ off,
+                                                       const int* __restrict__ total_cells,
                                                        const unsigned* __restrict__ flags) {
   __shared__ int run[256];      // per digit: where the next item of this workgroup goes
   __shared__ int wcnt[4][256];  // per wavefront of the round: items per digit
   if (fatal(flags)) return;
   const int tid = threadIdx.x;
+  if (pass_is_identity(total_cells, shift)) {
+    const int b1 = min(n, static_cast<int>(blockIdx.x) * kRadixTile + kRadixTile);
+    for (int i = blockIdx.x * kRadixTile + tid; i < b1; i += 256) {
+      kout[i] = kin[i];
+      vout[i] = vin[i];
+    }
+    return;
+  }
   const int lane = tid & (kWave - 1);
   const int w = tid >> 6;
   run[tid] = off[tid * gridDim.x + blockIdx.x];
@@ -2378,14 +2395,15 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
     for (int q = 0; q < passes; ++q) {
       const int i = q & 1, o = i ^ 1;
       k_radix_hist<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], 8 * q, c.radix_hist,
-                                                   c.flags);
+                                                   c.total_cells, c.flags);
       k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, nullptr, c.radix_len,
                                                    c.scan_partial, c.flags);
       k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.radix_hist, nullptr, c.radix_len,
                                                  c.scan_partial, c.radix_off, nullptr, 0,
                                                  c.flags);
       k_radix_scatter<<<c.radix_blocks, 256, 0, st>>>(c.n, c.sort_k[i], c.sort_v[i], c.sort_k[o],
-                                                      c.sort_v[o], 8 * q, c.radix_off, c.flags);
+                                                      c.sort_v[o], 8 * q, c.radix_off,
+                                                      c.total_cells, c.flags);
     }
   }));
   FH_TRY(L("reorder", [&] {
