@@ -46,7 +46,7 @@ STEP_BYTES_ALL_FLAGS = 480.0   # whole substep, all flags, 4 iterations
 FLOPS_PER_PAIR = {"lambda": 33.0, "delta": 30.0, "xsph": 25.0, "vorticity_omega": 31.0,
                   "vorticity_apply": 23.0}
 FP32_PEAK_TFLOPS = 157.3
-KERNELS = ["predict_bbox", "grid_setup", "cell_count", "scan", "sort", "reorder", "tiles", "runs",
+KERNELS = ["predict_bbox", "grid_setup", "cell_keys", "sort", "cell_count", "reorder", "tiles", "runs",
            "neighbors", "lambda", "delta", "xsph", "vorticity_omega", "vorticity_apply",
            "tile_neighbors", "tile_lambda", "tile_delta", "tile_xsph", "tile_vorticity_omega",
            "tile_vorticity_apply"]

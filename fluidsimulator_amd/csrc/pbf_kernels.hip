@@ -12,13 +12,14 @@
 //   predict_bbox  v' = v + g dt, pred0 = x + v' dt, cell = floor(pred0/h),
 //                 per-tile bounding boxes of cells        (core.cpp:150-161)
 //   grid_setup    per-scene table origin/extent, key offsets, counters
-//   cell_count    key of each particle, wavefront-aggregated cell histogram
-//   scan          exclusive scan of the histogram -> cell starts (two
-//                 launches: range sums, then each range scanned)
+//   cell_keys     key of each particle
 //   sort          keys in original-id order, then 8-bit LSD radix passes
 //                 (digit histogram, its scan, stable ballot-ranked scatter):
 //                 sorting stably by key leaves each cell's particles in
 //                 ascending id, the reference's order (core.cpp:164-203)
+//   cell_count    cell histogram from the sorted keys' runs, its exclusive
+//                 scan -> cell starts (two launches: range sums, then each
+//                 range scanned)
 //   reorder       gather each particle's state into its slot, recompute pred0
 //   tiles         with 4-cell bricks: the bricks that run through LDS and
 //                 their neighbor lists (pbf_tiles.hip; on the side stream in
@@ -422,30 +423,50 @@ constexpr int kCandPoints = 8;
 constexpr int kTilePoints = 2;
 
 
-__global__ __launch_bounds__(1024) void k_cell_count(StepConsts c,
-                                                     const uint64_t* __restrict__ cellc,
-                                                     const int* __restrict__ seg,
-                                                     const GridDesc* __restrict__ grid,
-                                                     int* __restrict__ count, int* __restrict__ key,
-                                                     const unsigned* __restrict__ flags) {
-  const int i0 = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
-  if (fatal(flags)) return;
-  const bool active = i0 < c.n;
-  const int i = active ? i0 : c.n - 1;
+// Each particle's cell key, in the previous substep's slot order.
+__global__ __launch_bounds__(1024) void k_cell_keys(StepConsts c,
+                                                    const uint64_t* __restrict__ cellc,
+                                                    const int* __restrict__ seg,
+                                                    const GridDesc* __restrict__ grid,
+                                                    int* __restrict__ key,
+                                                    const unsigned* __restrict__ flags) {
+  const int i = tile_index(c.xcd_remap) * blockDim.x + threadIdx.x;
+  if (fatal(flags) || i >= c.n) return;
   const GridDesc g = grid[scene_of(i, seg, c.n_scenes)];
   int x, y, z;
   unpack_cell(cellc[i], x, y, z);
-  const int k = cell_key(x - g.ox, y - g.oy, z - g.oz, g);
-  // Cell histogram, wavefront-aggregated: the particles arrive in the
-  // previous substep's sorted order, so neighboring lanes mostly share a
-  // cell, and each group of lanes with one key takes one atomic.
+  key[i] = cell_key(x - g.ox, y - g.oy, z - g.oz, g);
+}
+
+// The cell histogram from the sorted keys, where each cell's particles are
+// one run.  A run inside one wavefront stores its length; a run that crosses
+// a wavefront boundary adds each wavefront's part atomically.  A substep
+// thus takes about one atomic per wavefront instead of one per cell and
+// wavefront (device-scope atomics are carried out in memory on this
+// multi-XCD part).  The histogram is all zeros on entry: the scan that reads
+// it clears it.
+__global__ __launch_bounds__(1024) void k_cell_runs(int n, const int* __restrict__ sk,
+                                                    int* __restrict__ count,
+                                                    const unsigned* __restrict__ flags) {
+  if (fatal(flags)) return;  // uniform
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1);
-  for_each_key_group(active, k, [&](int leader, bool mine) {
-    const unsigned long long gm = __ballot(mine);
-    if (lane == leader) atomicAdd(count + k, __popcll(gm));
-  });
-  if (!active) return;
-  key[i] = k;
+  const bool active = s < n;
+  const int k = active ? sk[s] : 0;
+  const bool head = active && (s == 0 || sk[s - 1] != k);
+  const bool tail = active && (s == n - 1 || sk[s + 1] != k);
+  // segments: a run's slots within one wavefront; active lanes are a prefix
+  const unsigned long long starts = __ballot(active && (head || lane == 0));
+  const unsigned long long tails = __ballot(tail);
+  const int nact = __popcll(__ballot(active));
+  if (!active || ((starts >> lane) & 1ull) == 0ull) return;
+  const unsigned long long above = starts & ~((2ull << lane) - 1ull);
+  const int next = above != 0ull ? __builtin_ctzll(above) : nact;
+  const int len = next - lane;
+  if (head && ((tails >> (next - 1)) & 1ull) != 0ull)
+    count[k] = len;  // the whole run: its only writer
+  else
+    atomicAdd(count + k, len);
 }
 
 // ---- exclusive scan over a device-sized array (three launches, fixed grid)
@@ -2376,14 +2397,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                     c.flags, c.nbr_total, c.need_cells, c.steps_started,
                                     tiles ? c.tile_count : nullptr);
   }));
-  FH_TRY(L("cell_count", [&] {
-    k_cell_count<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.count, c.key, c.flags);
-  }));
-  FH_TRY(L("scan", [&] {
-    k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells,
-                                                 c.scan_partial, c.flags);
-    k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells, c.scan_partial,
-                                               c.start, nullptr, 1, c.flags);
+  FH_TRY(L("cell_keys", [&] {
+    k_cell_keys<<<nb, bs, 0, st>>>(k, c.cellc, c.seg, c.grid, c.key, c.flags);
   }));
   // keys lie below the cell table's capacity: 8 bits per pass
   int key_bits = 1;
@@ -2405,6 +2420,15 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
                                                       c.sort_v[o], 8 * q, c.radix_off,
                                                       c.total_cells, c.flags);
     }
+  }));
+  FH_TRY(L("cell_count", [&] {
+    // the histogram of the sorted keys and its scan: each cell's first slot
+    k_cell_runs<<<blocks_for(c.n, 1024), 1024, 0, st>>>(c.n, c.sort_k[passes & 1], c.count,
+                                                        c.flags);
+    k_scan_reduce<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells,
+                                                 c.scan_partial, c.flags);
+    k_scan_down<<<c.scan_blocks, 256, 0, st>>>(c.count, nullptr, c.total_cells, c.scan_partial,
+                                               c.start, nullptr, 1, c.flags);
   }));
   FH_TRY(L("reorder", [&] {
     k_reorder<<<nb, bs, 0, st>>>(k, c.sort_k[passes & 1], c.sort_v[passes & 1], c.seg, c.planes,

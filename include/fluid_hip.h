@@ -306,8 +306,8 @@ int fluid_hip_step_noalloc(const fluid_hip_params* params, int64_t n, float* pos
 /* ---- timing -------------------------------------------------------------- */
 
 /* Per-kernel device time (HIP events on the context's stream) of the steps
- * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_count",
- * "scan", "sort", "reorder", "tiles", "runs", "neighbors", "lambda", "delta", "xsph",
+ * queued while enabled.  Names: "predict_bbox", "grid_setup", "cell_keys",
+ * "sort", "cell_count", "reorder", "tiles", "runs", "neighbors", "lambda", "delta", "xsph",
  * "vorticity_omega", "vorticity_apply"; with LDS tiles also "tile_neighbors",
  * "tile_lambda", "tile_delta", "tile_xsph", "tile_vorticity_omega" and
  * "tile_vorticity_apply", the tile kernels alone, timed on the stream they run
