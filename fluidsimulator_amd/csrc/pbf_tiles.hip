@@ -429,6 +429,13 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
 // (in_tile = 0) and the passes skip it.  With nbr (a recorded substep) the
 // live tiles' lists are also stored there as slots (noff, nwords), as the
 // per-lane neighbor kernel stores its lists.
+// A halo record's p^0 and slot.
+struct RecSlot {
+  float3 p;
+  int t;
+};
+
+template <bool kStore>
 __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     StepConsts c, TileArgs ta, const float4* __restrict__ p0, int4* __restrict__ tile_rec,
     int* __restrict__ in_tile, int* __restrict__ ncount, long long* __restrict__ noff,
@@ -436,6 +443,7 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     unsigned long long* __restrict__ nbr_total, unsigned* __restrict__ flags) {
   __shared__ TileLayout L;
   __shared__ float X[kTileMaxHalo], Y[kTileMaxHalo], Z[kTileMaxHalo];
+  __shared__ int Sl[kStore ? kTileMaxHalo : 1];  // each record's slot (stored lists)
   __shared__ int scan_lds[kPass / 64 + 1];
   __shared__ long long base_sh;
   if (tile_fatal(flags)) return;
@@ -451,11 +459,12 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     const TileInfo T = tile_info(tile_rec, i);
     load_layout(L, ta.tile_seg, ta.tile_src, i);
     __syncthreads();
-    stage_halo(L, T.nrec, [&](int t) { return ld3(p0, t); },
-               [&](int r, const float3& v) {
-                 X[r] = v.x;
-                 Y[r] = v.y;
-                 Z[r] = v.z;
+    stage_halo(L, T.nrec, [&](int t) { return RecSlot{ld3(p0, t), t}; },
+               [&](int r, const RecSlot& v) {
+                 X[r] = v.p.x;
+                 Y[r] = v.p.y;
+                 Z[r] = v.p.z;
+                 if (kStore) Sl[r] = v.t;
                });
     __syncthreads();
     int over = 0;
@@ -516,10 +525,11 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     if (__syncthreads_or(over)) {
       for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) in_tile[s] = 0;
       if (threadIdx.x == 0) tile_rec[2 * i + 1].w = -1;
-    } else if (nbr) {
+    } else if (kStore) {
       // a recorded substep: the lists also go to the shared list buffer as
-      // plain slot words, in the same order, for the adjoint; one allocation
-      // per tile
+      // plain slot words, in the same order, for the adjoint (each list read
+      // back from its chunks, each record mapped to its slot); one
+      // allocation per tile
       int mine = 0;
       for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) mine += cnt_out[s];
       int total;
@@ -533,24 +543,17 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
         if (threadIdx.x == 0) atomicOr(flags, kFlagNeighbors);
       } else {
         for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) {
-          int lx, ly, lz;
-          local_cell(ta.slot_cell[s], T.cx0, T.cy0, T.cz0, lx, ly, lz);
-          const int own = ((lz + 1) * kHalo + ly + 1) * kHalo + lx + 1;
-          const int self = L.seg[own] + (s - L.src[own]);
-          const float3 me = make_float3(X[self], Y[self], Z[self]);
-          int k = 0;
-          for (int row = 0; row < 9; ++row) {
-            const int hc = ((lz + row / 3) * kHalo + ly + row % 3) * kHalo + lx;
-            for (int cc = hc; cc < hc + 3; ++cc) {
-              const int lo = L.seg[cc], hi = L.seg[cc + 1];
-              for (int r = lo; r < hi; ++r)
-                if (r != self && near0(me, make_float3(X[r], Y[r], Z[r]), c))
-                  nbr[off + k++] = L.src[cc] + (r - lo);
-            }
+          const int cnt = cnt_out[s];
+          for (int q8 = 0; q8 < cnt; q8 += 8) {
+            const uint4 ch = list[static_cast<size_t>(q8 >> 3) * c.n + s];
+            const unsigned w[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if (q8 + e < cnt) nbr[off + q8 + e] = Sl[(w[e >> 1] >> (16 * (e & 1))) & 0xffffu];
           }
           noff[s] = off;
-          nwords[s] = k;
-          off += k;
+          nwords[s] = cnt;
+          off += cnt;
         }
       }
     }
@@ -882,9 +885,14 @@ hipError_t enqueue_tile_classify(Ctx& c, const StepConsts& k) {
 
 hipError_t enqueue_tile_neighbors(Ctx& c, const StepConsts& k, const TileArgs& ta,
                                   const float4* p0, bool store, hipStream_t st) {
-  k_tile_neighbors<<<kTileGrid, kPass, 0, st>>>(
-      k, ta, p0, c.tile_rec, c.in_tile, c.ncount, c.noff, c.nwords, store ? c.nbr : nullptr,
-      c.nbr_cap, c.nbr_total, c.flags);
+  if (store)
+    k_tile_neighbors<true><<<kTileGrid, kPass, 0, st>>>(k, ta, p0, c.tile_rec, c.in_tile,
+                                                        c.ncount, c.noff, c.nwords, c.nbr,
+                                                        c.nbr_cap, c.nbr_total, c.flags);
+  else
+    k_tile_neighbors<false><<<kTileGrid, kPass, 0, st>>>(k, ta, p0, c.tile_rec, c.in_tile,
+                                                         c.ncount, c.noff, c.nwords, nullptr,
+                                                         c.nbr_cap, c.nbr_total, c.flags);
   return hipGetLastError();
 }
 
