@@ -27,7 +27,10 @@ constexpr int kTileMaxCand = 512;
 constexpr int kTileList = 512;
 constexpr int kTileListSmall = 256;
 static_assert(kTileList >= kTileMaxCand - 1, "large-batch tiles must never overflow");
-constexpr int kForkMinParticles = 1 << 18;  // tile kernels on the side stream from this size
+// large batches: tile criteria and list capacity, and the tile kernels on the
+// side stream (at 65,536 particles overlapping the two kernels of a pass
+// saves about what the fork and join cost)
+constexpr int kForkMinParticles = 1 << 18;
 constexpr int kTileAutoMinParticles = 1 << 15;  // default tile mode: tiles from this size
 constexpr int kTileGrid = 2048;     // workgroups of the persistent tile kernels
 

@@ -2257,10 +2257,14 @@ __global__ __launch_bounds__(1024) void k_run_fill(StepConsts c, const int* __re
                                                    const int* __restrict__ rep,
                                                    float4* __restrict__ a,
                                                    const float4* __restrict__ ids,
+                                                   const int* __restrict__ in_tile,
                                                    const unsigned* __restrict__ flags) {
   if (fatal(flags)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
+  // a tile computes each of its particles (coincident ones included), and
+  // its kernel may run beside this one
+  if (in_tile && in_tile[s]) return;
   const int h = rep[run_no[s]];
   if (h == s) return;
   float4 v = a[h];
@@ -2371,8 +2375,8 @@ hipError_t enqueue_substep(Ctx& c, const SubstepRec* rec) {
   c.last_runs = runs;
   auto fill = [&](float4* a, const float4* ids) {
     if (runs)
-      k_run_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(k, c.run_no, c.rep_list, a, ids,
-                                                             c.flags);
+      k_run_fill<<<blocks_for(c.n, 256), 256, 0, c.stream>>>(
+          k, c.run_no, c.rep_list, a, ids, tiles ? c.in_tile : nullptr, c.flags);
   };
   const int bs = c.block;
   const int nb = blocks_for(c.n, bs);
