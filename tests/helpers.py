@@ -36,6 +36,21 @@ def same_bits(a, b):
     return np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def same_bits_or_nan(a, b):
+    """Bitwise equality of float arrays, except that a NaN equals any NaN at
+    the same position: IEEE 754 leaves the sign and payload of an arithmetic
+    NaN result open, and two kernels that compute the same operations in the
+    same order may still pass a different NaN operand through."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.uint8), b[~nb].view(np.uint8))
+
+
 def neighbor_lists(counts, flat):
     off = np.concatenate([[0], np.cumsum(counts)])
     return [flat[off[i]:off[i + 1]] for i in range(len(counts))]
