@@ -31,8 +31,10 @@
 //   gravity  per-scene sum of the predict's g-bar (fixed order, double)
 // The recorded forward supplies p^k (with lambda^k), S and Q, the plane and
 // restitution decisions and the lists; nothing is recomputed here.
-// Every sum runs in list order and the gravity sum in a fixed tree, so the
-// gradients are deterministic (bitwise repeatable).
+// Each gather splits a particle's list over kSplit lanes in a fixed
+// interleave and adds their partial sums in a fixed order, and the gravity
+// sum runs in a fixed tree, so the gradients are deterministic (bitwise
+// repeatable, and independent of how a list is stored).
 
 #include "pbf_ctx.h"
 #include "pbf_device.h"
@@ -44,6 +46,42 @@ namespace {
 __device__ __forceinline__ float3 f3(const float4& v) { return make_float3(v.x, v.y, v.z); }
 __device__ __forceinline__ float dot3(const float3& a, const float3& b) {
   return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// The gathers give each particle kSplit consecutive lanes: lane q of a
+// group takes list entries q, q + kSplit, q + 2 kSplit, ... in list order
+// (every lane decodes the whole run-length list), so a particle's chain of
+// dependent loads is a quarter as long and four times as many wavefronts
+// hide each other's latency.  group_sum adds the group's partial sums in
+// one fixed order (lanes 0+1 and 2+3, then the two pairs) and gives every
+// lane of the group the total.
+constexpr int kSplit = 4;
+
+template <typename Load, typename F>
+__device__ __forceinline__ void for_each_neighbor_split(const NbrView& v, int s, int q,
+                                                        Load&& load, F&& f) {
+  const int* __restrict__ L = v.list + v.off[s];
+  const int nw = v.nw[s];
+  int idx = 0;  // list index of the run's first entry
+  for (int w = 0; w < nw;) {
+    const int e = L[w++];
+    const int t0 = e & 0x7fffffff;
+    const int len = e < 0 ? L[w++] : 1;
+    // the first entry of this run whose list index is q modulo kSplit
+    const int skip = (q - idx) & (kSplit - 1);
+    for (int t = t0 + skip; t < t0 + len; t += kSplit) f(t, load(t));
+    idx += len;
+  }
+}
+
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int m = 1; m < kSplit; m <<= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ float3 group_sum(float3 v) {
+  return make_float3(group_sum(v.x), group_sum(v.y), group_sum(v.z));
 }
 
 // Geometry of a pair with r2 < h2: the spiky factor F (clamped below
@@ -169,16 +207,17 @@ __global__ __launch_bounds__(256) void k_adj_xsph(
     const float4* __restrict__ U, const unsigned* __restrict__ Mlast, const int* __restrict__ seg,
     const ScenePlanes* __restrict__ sp, float4* __restrict__ PB, float4* __restrict__ XB,
     float4* __restrict__ Wv, float* __restrict__ RB) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gi / kSplit, qi = gi & (kSplit - 1);
+  if (s >= c.n) return;  // uniform over a lane group
   const float3 xi = ld3(PI, s);
   const float4 vi = VC[s];
   const float3 ui = ld3(U, s);
   const float mui = (vi.w > 0.0f) ? (c.mass / vi.w) : 0.0f;
   float3 av = make_float3(0.f, 0.f, 0.f), ax = make_float3(0.f, 0.f, 0.f);
   float ar = 0.0f;
-  for_each_neighbor(
-      nv, s, [&](int t) { return XNb{ld3(PI, t), VC[t], ld3(U, t)}; },
+  for_each_neighbor_split(
+      nv, s, qi, [&](int t) { return XNb{ld3(PI, t), VC[t], ld3(U, t)}; },
       [&](int, const XNb& q) {
         const float3 d = make_float3(xi.x - q.x.x, xi.y - q.x.y, xi.z - q.x.z);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
@@ -202,6 +241,10 @@ __global__ __launch_bounds__(256) void k_adj_xsph(
         // rho-bar_i: W u_j . (v_i - v_j)
         ar -= W * dot3(q.u, dv);
       });
+  av = group_sum(av);
+  ax = group_sum(ax);
+  ar = group_sum(ar);
+  if (qi != 0) return;
   const float cv = c.visc_c;
   const float3 vcbar = make_float3(ui.x + cv * av.x, ui.y + cv * av.y, ui.z + cv * av.z);
   const float4 pb = PB[s];
@@ -223,14 +266,15 @@ __global__ __launch_bounds__(256) void k_adj_delta(StepConsts c, NbrView nv,
                                                    const float4* __restrict__ SQ,
                                                    float4* __restrict__ PB, float* __restrict__ RB,
                                                    float4* __restrict__ LB, int add_rb) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gi / kSplit, qi = gi & (kSplit - 1);
+  if (s >= c.n) return;  // uniform over a lane group
   const float4 pi = P[s];
   const float3 wi = ld3(Wv, s);
   float lb = 0.0f;
   float3 gp = make_float3(0.f, 0.f, 0.f);
-  for_each_neighbor(
-      nv, s, [&](int t) { return DNb{P[t], ld3(Wv, t)}; },
+  for_each_neighbor_split(
+      nv, s, qi, [&](int t) { return DNb{P[t], ld3(Wv, t)}; },
       [&](int, const DNb& q) {
         const float3 d = make_float3(pi.x - q.p.x, pi.y - q.p.y, pi.z - q.p.z);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
@@ -248,6 +292,9 @@ __global__ __launch_bounds__(256) void k_adj_delta(StepConsts c, NbrView nv,
         gp.y += a * u.y + b * d.y;
         gp.z += a * u.z + b * d.z;
       });
+  lb = group_sum(lb);
+  gp = group_sum(gp);
+  if (qi != 0) return;
   // lambda = -C / (Q + eps), C = rho/rho0 - 1
   const float4 sq = SQ[s];
   const float den = sq.w + c.eps;
@@ -272,16 +319,17 @@ __global__ __launch_bounds__(256) void k_adj_lambda(
     float4* __restrict__ Wv, const float4* __restrict__ X, const float4* __restrict__ XB,
     float4* __restrict__ GX, float4* __restrict__ GV, const int* __restrict__ seg,
     const ScenePlanes* __restrict__ sp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
+  const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gi / kSplit, qi = gi & (kSplit - 1);
+  if (s >= c.n) return;  // uniform over a lane group
   const float3 pi = ld3(P, s);
   const float4 li = LB[s];
   const float ri = RB[s];
   const float g2 = 2.0f * c.grad_scale * c.grad_scale;
   const float m2 = 2.0f * c.mass;
   float3 gp = make_float3(0.f, 0.f, 0.f);
-  for_each_neighbor(
-      nv, s, [&](int t) { return LNb{ld3(P, t), LB[t], RB[t]}; },
+  for_each_neighbor_split(
+      nv, s, qi, [&](int t) { return LNb{ld3(P, t), LB[t], RB[t]}; },
       [&](int, const LNb& q) {
         const float3 d = make_float3(pi.x - q.p.x, pi.y - q.p.y, pi.z - q.p.z);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
@@ -298,6 +346,8 @@ __global__ __launch_bounds__(256) void k_adj_lambda(
         gp.y += g.F * u.y + b * d.y;
         gp.z += g.F * u.z + b * d.z;
       });
+  gp = group_sum(gp);
+  if (qi != 0) return;
   const float4 pb0 = PB[s];
   const float3 pb = make_float3(pb0.x + gp.x, pb0.y + gp.y, pb0.z + gp.z);
   if (Mprev) {
@@ -443,6 +493,7 @@ hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b
   const int I = c.iterations;
   const int n = c.n;
   const int nb = nblocks(n, 256);
+  const int ng = nblocks(static_cast<long long>(n) * kSplit, 256);  // the gathers' lane groups
   hipStream_t st = c.stream;
   const NbrView nv{t.noff, t.nwords, t.nbr, t.ncount};
   const size_t N = static_cast<size_t>(n);
@@ -451,15 +502,15 @@ hipError_t enqueue_adjoint_substep(Ctx& c, const SubTape& t, const AdjBuffers& b
                                  c.planes, b.GX, b.GV, b.U, b.PB, b.XB, b.W, b.RB,
                                  c.xsph ? 1 : 0);
   if (c.xsph)
-    k_adj_xsph<<<nb, 256, 0, st>>>(k, nv, t.PI, t.VC, b.U, Mlast, c.seg, c.planes, b.PB, b.XB,
+    k_adj_xsph<<<ng, 256, 0, st>>>(k, nv, t.PI, t.VC, b.U, Mlast, c.seg, c.planes, b.PB, b.XB,
                                    b.W, b.RB);
   for (int it = I - 1; it >= 0; --it) {
     const float4* P = t.P + static_cast<size_t>(it) * N;
     const float4* SQ = t.SQ + static_cast<size_t>(it) * N;
-    k_adj_delta<<<nb, 256, 0, st>>>(k, nv, P, b.W, SQ, b.PB, b.RB, b.LB,
+    k_adj_delta<<<ng, 256, 0, st>>>(k, nv, P, b.W, SQ, b.PB, b.RB, b.LB,
                                     (it == I - 1 && c.xsph) ? 1 : 0);
     const unsigned* Mprev = it > 0 ? t.M + static_cast<size_t>(it - 1) * N : nullptr;
-    k_adj_lambda<<<nb, 256, 0, st>>>(k, nv, P, b.LB, b.RB, b.PB, Mprev, b.W, t.X, b.XB, b.GX,
+    k_adj_lambda<<<ng, 256, 0, st>>>(k, nv, P, b.LB, b.RB, b.PB, Mprev, b.W, t.X, b.XB, b.GX,
                                      b.GV, c.seg, c.planes);
   }
   k_adj_gravity<<<c.n_scenes, 1024, 0, st>>>(k, c.seg, b.PB, b.gacc);
