@@ -1385,10 +1385,15 @@ int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n) {
                           hipMemcpyDeviceToHost),
                 "read tiles");
       for (int i = 0; i < tiles; ++i)
-        if (rec[2 * i + 1].w >= 0) {  // live: not handed back to the per-lane passes
-          ++v[8];
-          v[9] += rec[2 * i].y - rec[2 * i].x;
-        }
+        if (rec[2 * i + 1].w >= 0) ++v[8];  // live: not handed back to the per-lane passes
+      // the particles the tiles computed (a tile leaves its brick's dense or
+      // coincident cells to the per-lane passes)
+      std::vector<int> in_tile(static_cast<size_t>(c.n));
+      if (c.n)
+        HIP_TRY(hipMemcpy(in_tile.data(), c.in_tile, sizeof(int) * in_tile.size(),
+                          hipMemcpyDeviceToHost),
+                "read tiles");
+      for (int x : in_tile) v[9] += x > 0;
     }
   }
   for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];

@@ -4,17 +4,20 @@
 // brick's particles are one contiguous slot range and the 6x6x6 cells around
 // it (the brick and a one-cell halo) hold every candidate of those particles.
 // A brick of enough particles (kTileMinParticles in large batches) is a tile
-// when its whole halo holds at most kTileMaxHalo particles, each of its
-// non-empty cells' 27-cell neighborhoods fewer than kTileMaxCand, in large
-// batches no two of its particles coincide (the per-lane passes compute a run
-// of coincident particles once) and, found by the neighbor kernel, none of
-// its particles has more neighbors than a tile list holds (kTileList, or
-// kTileListSmall in small batches).  For a tile:
+// when its whole halo holds at most kTileMaxHalo particles, one of its cells
+// is good and, found by the neighbor kernel, none of its particles has more
+// neighbors than a tile list holds (kTileList, or kTileListSmall in small
+// batches).  A cell is good when its 27-cell neighborhood holds fewer than
+// kTileMaxCand candidates and, in large batches, no two of its particles
+// coincide (the per-lane passes compute a run of coincident particles once).
+// The tile computes the particles of its good cells; the other cells of the
+// brick go to the per-lane passes, their particles staged in the halo like
+// any other.  For a tile:
 //
 //   * classify (once per substep): records the tile, the LDS offset of each
 //     halo cell (the cells' particle counts, scanned in halo order z, y, x)
-//     and each halo cell's first slot, and marks the brick's slots so that
-//     the lane map of the per-lane passes leaves them out;
+//     and each halo cell's first slot, and marks the good cells' slots so
+//     that the lane map of the per-lane passes leaves them out;
 //   * neighbors (once per substep; in small batches before the lane map, in
 //     large ones, whose lists cannot overflow, beside the lanes' lists):
 //     stages the halo's p^0 in LDS and writes each particle's neighbor list
@@ -182,7 +185,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_bricks(
 
 // Examines the candidate bricks (one workgroup at a time, persistent grid):
 // a tile gets a record, its halo layout and in_tile[s] = 1 + its index for
-// its slots; the other candidates' slots get in_tile = 0.
+// the slots of its good cells; every other slot of a candidate gets
+// in_tile = 0.
 __global__ __launch_bounds__(kThreads) void k_tile_classify(
     StepConsts c, const GridDesc* __restrict__ grid, const int* __restrict__ start,
     const float4* __restrict__ posB, const int* __restrict__ tile_cand,
@@ -192,6 +196,8 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
   __shared__ int hcnt[kHaloCells];
   __shared__ int hsrc[kHaloCells];
   __shared__ int scan_lds[kThreads / 64 + 1];
+  __shared__ int cst[65];   // the brick's cell starts (slots)
+  __shared__ int cbad[64];  // per brick cell: left to the per-lane passes
   __shared__ int idx_sh;
   if (tile_fatal(flags)) return;
   const int ncand = min(tile_count[1], tile_cap);
@@ -217,32 +223,51 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
       hcnt[t] = cnt;
       hsrc[t] = src;
     }
+    if (t <= 64) cst[t] = start[(b << 6) + t];
     __syncthreads();
     int total;
     const int off = tile_excl_scan(cnt, scan_lds, &total);
     // a cell whose 27-cell neighborhood holds kTileMaxCand candidates or more
-    bool bad = false;
+    // (brick cell t is the cell of key (b << 6) + t: Z-order, bits
+    // x0 y0 z0 x1 y1 z1)
     if (t < 64) {
-      // the brick's cell (x, y, z) = (t & 3, (t >> 2) & 3, t >> 4)
-      const int hx = (t & 3) + 1, hy = ((t >> 2) & 3) + 1, hz = (t >> 4) + 1;
-      if (hcnt[(hz * kHalo + hy) * kHalo + hx] > 0) {
-        int cand = 0;
-        for (int dz = -1; dz <= 1; ++dz)
-          for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx)
-              cand += hcnt[((hz + dz) * kHalo + hy + dy) * kHalo + hx + dx];
-        bad = cand >= kTileMaxCand;
+      const int hx = ((t & 1) | ((t >> 2) & 2)) + 1;
+      const int hy = (((t >> 1) & 1) | ((t >> 3) & 2)) + 1;
+      const int hz = (((t >> 2) & 1) | ((t >> 4) & 2)) + 1;
+      int cand = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx)
+            cand += hcnt[((hz + dz) * kHalo + hy + dy) * kHalo + hx + dx];
+      cbad[t] = cand >= kTileMaxCand;
+    }
+    __syncthreads();
+    // coincident particles (consecutive slots at one position, so in one
+    // cell): in large batches the per-lane passes, which compute a run of
+    // them once, take their cell.  Only a brick that can still be a tile
+    // (at most kTileMaxHalo slots) is scanned; each thread keeps the cell of
+    // its slot (cells lie in slot order) and jumps over excluded cells.
+    const bool maybe =
+        __syncthreads_or(t < 64 && cst[t + 1] > cst[t] && !cbad[t]) && total <= kTileMaxHalo;
+    if (reject_runs && maybe) {
+      int q = 0;
+      for (int s = a + 1 + t; s < e; s += kThreads) {
+        while (cst[q + 1] <= s) ++q;
+        if (cbad[q]) {  // left to the per-lane passes: on to the next cell
+          s += (cst[q + 1] - s - 1) / kThreads * kThreads;
+          continue;
+        }
+        if (coincide(ld3(posB, s), ld3(posB, s - 1))) cbad[q] = 1;
       }
     }
-    // coincident particles (consecutive slots at one position): in large
-    // batches the per-lane passes, which compute a run of them once, take them
-    if (reject_runs)
-      for (int s = a + 1 + t; s < e && !bad; s += kThreads)
-        bad = coincide(ld3(posB, s), ld3(posB, s - 1));
-    const bool ok = !__syncthreads_or(bad) && total <= kTileMaxHalo;
+    __syncthreads();
+    const bool good = t < 64 && cst[t + 1] > cst[t] && !cbad[t];
+    const bool ok = __syncthreads_or(good) && maybe;
     if (ok && t == 0) idx_sh = atomicAdd(tile_count, 1);
     __syncthreads();
     const bool tile = ok && idx_sh < tile_cap;
+    // a tile with excluded cells marks its slots cell by cell
+    const bool mixed = __syncthreads_or(tile && t < 64 && cst[t + 1] > cst[t] && cbad[t]);
     if (tile) {
       const int i = idx_sh;
       if (t == 0) {
@@ -255,7 +280,15 @@ __global__ __launch_bounds__(kThreads) void k_tile_classify(
       }
       if (t == 0) tile_seg[i * (kHaloCells + 1) + kHaloCells] = total;
     }
-    for (int s = a + t; s < e; s += kThreads) in_tile[s] = tile ? idx_sh + 1 : 0;
+    if (mixed) {
+      int q = 0;
+      for (int s = a + t; s < e; s += kThreads) {
+        while (cst[q + 1] <= s) ++q;
+        in_tile[s] = cbad[q] ? 0 : idx_sh + 1;
+      }
+    } else {
+      for (int s = a + t; s < e; s += kThreads) in_tile[s] = tile ? idx_sh + 1 : 0;
+    }
     __syncthreads();  // LDS is reused by the next brick
   }
 }
@@ -367,7 +400,8 @@ __device__ __forceinline__ int tiles_of(const int* tile_count, int tile_cap) {
   return t < tile_cap ? t : tile_cap;
 }
 
-// A tile particle's neighbor count and first list chunk.
+// A tile particle's neighbor count (-1 for a slot of the tile's brick that
+// the per-lane passes compute) and first list chunk.
 struct Nbrs {
   int cnt;
   uint4 first;  // the list's first chunk
@@ -408,7 +442,7 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
     stage_halo(L, T.nrec, fetch, store);
     __syncthreads();
     while (s < T.e) {
-      body(s, T, nb, x);
+      if (nb.cnt >= 0) body(s, T, nb, x);  // count -1: a cell of the per-lane passes
       s += kPass;
       if (s < T.e) {
         nb = load_nbrs(ta, n, s);
@@ -426,7 +460,8 @@ __device__ __forceinline__ void tile_loop(const TileArgs& ta, int n, int xcd, Ti
 // each cell by ascending slot (= ascending id); neighbors are the other
 // candidates j with near0(p^0_i, p^0_j).  A tile with a particle of more
 // neighbors than its list holds is dead: its slots go back to the per-lane passes
-// (in_tile = 0) and the passes skip it.  With nbr (a recorded substep) the
+// (in_tile = 0) and the passes skip it.  The slots of the brick's other
+// cells get the count -1.  With nbr (a recorded substep) the
 // live tiles' lists are also stored there as slots (noff, nwords), as the
 // per-lane neighbor kernel stores its lists.
 // A halo record's p^0 and slot.
@@ -469,6 +504,10 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
     __syncthreads();
     int over = 0;
     for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) {
+      if (in_tile[s] == 0) {  // a cell of the per-lane passes: they write its lists
+        cnt_out[s] = -1;
+        continue;
+      }
       int lx, ly, lz;
       local_cell(ta.slot_cell[s], T.cx0, T.cy0, T.cz0, lx, ly, lz);
       const int own = ((lz + 1) * kHalo + ly + 1) * kHalo + lx + 1;
@@ -531,7 +570,8 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
       // back from its chunks, each record mapped to its slot); one
       // allocation per tile
       int mine = 0;
-      for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) mine += cnt_out[s];
+      for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass)
+        mine += max(cnt_out[s], 0);
       int total;
       const int pre = tile_excl_scan<kPass>(mine, scan_lds, &total);
       if (threadIdx.x == 0)
@@ -544,6 +584,7 @@ __global__ __launch_bounds__(kPass) void k_tile_neighbors(
       } else {
         for (int s = T.a + static_cast<int>(threadIdx.x); s < T.e; s += kPass) {
           const int cnt = cnt_out[s];
+          if (cnt < 0) continue;  // not the tile's
           for (int q8 = 0; q8 < cnt; q8 += 8) {
             const uint4 ch = list[static_cast<size_t>(q8 >> 3) * c.n + s];
             const unsigned w[4] = {ch.x, ch.y, ch.z, ch.w};

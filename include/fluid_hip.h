@@ -214,7 +214,7 @@ int fluid_hip_get_grid(fluid_hip_ctx* ctx, int32_t* slot_keys, int32_t* desc);
  * context was created, out[6] device bytes held by the rollout recording
  * (checkpoints and tapes), out[7] substeps the last backward pass
  * recomputed, out[8] bricks the last substep ran through the LDS-tiled
- * passes, out[9] particles of those bricks.  `n` entries are written
+ * passes, out[9] particles those tiles computed.  `n` entries are written
  * (<= 10). */
 int fluid_hip_get_stats(fluid_hip_ctx* ctx, int64_t* out, int32_t n);
 

@@ -1,13 +1,15 @@
 """The LDS-tiled passes of sparse bricks (pbf_tiles.hip) against the CPU
 oracle and against the per-lane passes, bitwise.
 
-A brick of 4x4x4 cells runs through LDS when every cell of it has fewer than
-512 candidates (particles in its 27 cells), its 6^3-cell halo holds at most
-2,048 particles and no particle's list outgrows the tile list (256 entries
-below 2^18 particles, 128 from there on, where a brick also needs 32 particles
-and no coincident pair); the others take the per-lane passes in the same
-substep.  Every sum visits the neighbors in the reference's order
-(core.cpp:205-247), so the mixed substep must equal the oracle bit for bit."""
+A brick of 4x4x4 cells runs through LDS when its 6^3-cell halo holds at most
+2,048 particles, one of its cells is good and no particle's list outgrows
+the tile list (256 entries below 2^18 particles, 512 from there on, where a
+brick also needs 32 particles).  A good cell has fewer than 512 candidates
+(particles in its 27 cells) and, from 2^18 particles on, no coincident pair;
+the brick's other cells, and the bricks that are no tile, take the per-lane
+passes in the same substep.  Every sum visits the neighbors in the
+reference's order (core.cpp:205-247), so the mixed substep must equal the
+oracle bit for bit."""
 import numpy as np
 import pytest
 
@@ -146,3 +148,64 @@ def test_brick_edges_without_tiles_agree():
         r, _, st = run(sc.params, sc.state.pos, sc.state.vel, 4, brick_edge=edge, tile_gather=1)
         assert st["tiles"] == 0
         assert same_bits(r.pos, ref.pos) and same_bits(r.vel, ref.vel)
+
+
+def _large_batch_scene(pair=False, cluster=False):
+    """64^3 particles at spacing 0.0625 (h = 0.1: about 4 per cell, 2^18 in
+    all) floating in a 6 m box.  `pair` moves particle a+1 onto particle a
+    (one coincident pair in consecutive slots); `cluster` adds 450 particles
+    in a 0.01 m cube inside the block, so the cluster's cell and its 26
+    neighbors have more than 512 candidates while the halos stay under
+    2,048."""
+    nx = 64
+    pos = scenes.lattice_block(nx, nx, nx, d=0.0625, origin=(0.5, 0.5, 0.5), seed=31)
+    if pair:
+        a = (32 * nx + 32) * nx + 32
+        pos[:, a + 1] = pos[:, a]
+    if cluster:
+        c0 = pos[:, (20 * nx + 20) * nx + 44].astype(np.float64)
+        u = scenes.uniform01(32, 3 * 450).reshape(3, 450)
+        pos = np.concatenate([pos, (c0[:, None] + 0.01 * (u - 0.5)).astype(np.float32)], axis=1)
+    return scenes.base_params((6.0, 6.0, 6.0), True), pos
+
+
+def test_large_batch_tiles_leave_only_dense_and_coincident_cells_to_the_lanes():
+    """From 2^18 particles on, a cell with a coincident pair or with 512
+    candidates or more goes to the per-lane passes and the rest of its brick
+    stays a tile: a pair costs the tiles its one cell (about 4 particles,
+    where its brick holds about 260), the cluster its 27 cells.  Tiles on,
+    tiles off and the oracle agree bit for bit."""
+    untiled = {}
+    for name, kw in (("plain", {}), ("pair", {"pair": True}), ("cluster", {"cluster": True})):
+        p, pos = _large_batch_scene(**kw)
+        vel = np.zeros_like(pos)
+        on, d_on, st_on = run(p, pos, vel, 1, tile_gather=1)
+        off, d_off, st_off = run(p, pos, vel, 1, tile_gather=2)
+        o = oracle.step(p, pos, vel, nsteps=1, diag="light")
+        untiled[name] = pos.shape[1] - st_on["tiled_particles"]
+        print(f"{name}: n {pos.shape[1]}, tiles {st_on['tiles']}, "
+              f"untiled {untiled[name]}", flush=True)
+        assert st_on["tiles"] > 0 and st_off["tiles"] == 0
+        for r, d in ((on, d_on), (off, d_off)):
+            assert same_bits(r.pos, o["pos"]), name
+            assert same_bits(r.vel, o["vel"]), name
+            assert same_bits(d["rho"], o["rho"]), name
+            assert np.array_equal(d["nbr_count"], o["nbr_count"]), name
+    assert 2 <= untiled["pair"] - untiled["plain"] <= 40
+    assert 450 < untiled["cluster"] - untiled["plain"] <= 450 + 64 * 12
+
+
+def test_large_batch_mixed_bricks_over_substeps():
+    """The pair and the cluster together over 3 substeps (the cluster blows
+    apart): tiles on equal the oracle bit for bit."""
+    p, pos = _large_batch_scene(pair=True, cluster=True)
+    vel = np.zeros_like(pos)
+    steps = 3
+    on, d_on, st_on = run(p, pos, vel, steps, tile_gather=1)
+    o = oracle.step(p, pos, vel, nsteps=steps, diag="light")
+    print(f"tiles {st_on['tiles']}, tiled {st_on['tiled_particles']} of {pos.shape[1]}", flush=True)
+    assert st_on["tiles"] > 0
+    assert same_bits(on.pos, o["pos"])
+    assert same_bits(on.vel, o["vel"])
+    assert same_bits(d_on["rho"], o["rho"])
+    assert np.array_equal(d_on["nbr_count"], o["nbr_count"])
