@@ -298,7 +298,8 @@ class HipSolver:
         cells, list capacity (words), cell capacity, capacity growths; device
         bytes of the rollout recording, substeps the last backward pass
         recomputed, and the bricks the last substep ran as LDS tiles and
-        their particles."""
+        the particles those tiles computed (a tile leaves its brick's dense
+        or coincident cells to the per-lane passes)."""
         out = (C.c_int64 * 10)()
         _check(self.lib.fluid_hip_get_stats(self.h, out, 10))
         keys = ("pairs", "words", "cells", "list_capacity", "cell_capacity", "grow_events",
