@@ -187,7 +187,7 @@ class HipSolver:
     counts: particle count per scene (defaults to [state.n] via upload)."""
 
     def __init__(self, params, counts, *, cell_capacity=0, neighbor_capacity=0, block_size=256,
-                 xcd_remap=True, record_diag=False, device=0, wave_mode=0, brick_edge=0,
+                 xcd_remap=False, record_diag=False, device=0, wave_mode=0, brick_edge=0,
                  lane_capacity=0, tile_gather=0):
         self.lib = load_library()
         plist = list(params) if isinstance(params, (list, tuple)) else [params]

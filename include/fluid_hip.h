@@ -88,7 +88,9 @@ typedef struct fluid_hip_config {
   int64_t neighbor_capacity;  /* neighbor-list entries, all scenes */
   int32_t device;             /* HIP device ordinal */
   int32_t block_size;         /* threads per workgroup of the particle kernels (64..1024) */
-  int32_t xcd_remap;          /* 1: map consecutive particle tiles to one XCD */
+  int32_t xcd_remap;          /* 1: map consecutive particle tiles to one XCD
+                                 (the tile, bounding-box and key kernels; 0, the
+                                 default, measured as fast or faster) */
   int32_t record_diag;        /* 1: keep per-step grid diagnostics (slower) */
   int32_t wave_mode;          /* gathers of wavefronts that share one cell: 0 default (2),
                                  1 per-lane lists only, 2 shared candidate tiles,
