@@ -21,9 +21,9 @@
 //                 scan -> cell starts (two launches: range sums, then each
 //                 range scanned)
 //   reorder       gather each particle's state into its slot, recompute pred0
-//   tiles         with 4-cell bricks: the bricks that run through LDS and
-//                 their neighbor lists (pbf_tiles.hip; on the side stream in
-//                 large batches)
+//   tiles         with 4-cell bricks: the bricks that run through LDS (the
+//                 particles of their good cells) and their neighbor lists
+//                 (pbf_tiles.hip; on the side stream in large batches)
 //   runs          runs of identical consecutive particles and the lane map of
 //                 the gather passes (dense cells from a wave boundary; tile
 //                 slots get no lane)
